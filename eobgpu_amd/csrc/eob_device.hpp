@@ -1,0 +1,245 @@
+// Device-side building blocks: per-binary data, the lane-group ODE right-hand
+// side, the DOP853 attempted step and the per-binary integration state machine
+// shared by the persistent (work-stealing) and the plain (one launch per step)
+// integrators.
+//
+// Lane groups.  A binary is integrated by a group of G consecutive lanes of one
+// wavefront (G = 1, 4, 8, 16, 32 or 64).  Inside the right-hand side
+//   - lanes 0..3 each evaluate H with one forward-mode tangent:
+//       lane 0 d/dr, lane 1 d/dpr, lane 2 d/dpphi, lane 3 d/dpphi at pr = 0
+//     (the reference evaluates two full gradients and a third H per RHS,
+//      src/EOB.cu:3102, :2612, :2619; the values are identical);
+//   - the 35 flux modes are spread over the G lanes and reduced with shuffles;
+//   - lane j < 4 owns state component j of the DOP853 stages.
+// With G = 1 one lane does all of it (the reference's lane-per-binary layout).
+#pragma once
+#include "eob_ham.hpp"
+#include "eob_flux.hpp"
+
+namespace eob {
+
+struct BinData {
+    HamCoef h;
+    WaveCoef w;
+    double eta_f;     // eta as the flux sees it (clamped to 1/4 within 1e-4)
+    double nu;        // mu / M as RR_force computes it
+};
+
+constexpr int kNAdd = 80;   // rows of additionalArgs
+
+// DOP853 tableau (Hairer & Wanner; same published coefficients as
+// scipy.integrate DOP853 and src/DOPR853.cu:11-185)
+__constant__ static double dC[12] = {0.0, 0.05260015195876773, 0.0789002279381516, 0.1183503419072274, 0.2816496580927726, 0.3333333333333333, 0.25, 0.3076923076923077, 0.6512820512820513, 0.6, 0.8571428571428571, 1.0};
+#define DOP_A                                                                                                                                   \
+    {{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                                                                              \
+     {0.05260015195876773, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                                                              \
+     {0.0197250569845379, 0.0591751709536137, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                                               \
+     {0.02958758547680685, 0.0, 0.08876275643042054, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                                             \
+     {0.2413651341592667, 0.0, -0.8845494793282861, 0.924834003261792, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                                \
+     {0.037037037037037035, 0.0, 0.0, 0.17082860872947386, 0.12546768756682242, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                           \
+     {0.037109375, 0.0, 0.0, 0.17025221101954405, 0.06021653898045596, -0.017578125, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},                           \
+     {0.03709200011850479, 0.0, 0.0, 0.17038392571223998, 0.10726203044637328, -0.015319437748624402, 0.008273789163814023, 0.0, 0.0, 0.0, 0.0, 0.0}, \
+     {0.6241109587160757, 0.0, 0.0, -3.3608926294469414, -0.868219346841726, 27.59209969944671, 20.154067550477894, -43.48988418106996, 0.0, 0.0, 0.0, 0.0}, \
+     {0.47766253643826434, 0.0, 0.0, -2.4881146199716677, -0.590290826836843, 21.230051448181193, 15.279233632882423, -33.28821096898486, -0.020331201708508627, 0.0, 0.0, 0.0}, \
+     {-0.9371424300859873, 0.0, 0.0, 5.186372428844064, 1.0914373489967295, -8.149787010746927, -18.52006565999696, 22.739487099350505, 2.4936055526796523, -3.0467644718982196, 0.0, 0.0}, \
+     {2.273310147516538, 0.0, 0.0, -10.53449546673725, -2.0008720582248625, -17.9589318631188, 27.94888452941996, -2.8589982771350235, -8.87285693353063, 12.360567175794303, 0.6433927460157636, 0.0}}
+constexpr double kB1 = 0.054293734116568765, kB6 = 4.450312892752409, kB7 = 1.8915178993145003, kB8 = -5.801203960010585,
+                 kB9 = 0.3111643669578199, kB10 = -0.1521609496625161, kB11 = 0.20136540080403034, kB12 = 0.04471061572777259;
+constexpr double kE1 = 0.01312004499419488, kE6 = -1.2251564463762044, kE7 = -0.4957589496572502, kE8 = 1.6643771824549864,
+                 kE9 = -0.35032884874997366, kE10 = 0.3341791187130175, kE11 = 0.08192320648511571, kE12 = -0.022355307863886294;
+constexpr double kBHH1 = 0.2440944881889764, kBHH2 = 0.7338466882816118, kBHH3 = 0.022058823529411766;
+constexpr double kSafe = 0.9, kAlpha = 0.125, kMinScale = 1.0 / 3.0, kMaxScale = 6.0;
+
+// ------------------------------------------------------------ group helpers
+template <int G>
+__device__ __forceinline__ double gshfl(double v, int src) {
+    if constexpr (G == 1) return v;
+    else return __shfl(v, src, G);
+}
+template <int G>
+__device__ __forceinline__ int gshfl_i(int v, int src) {
+    if constexpr (G == 1) return v;
+    else return __shfl(v, src, G);
+}
+template <int G>
+__device__ __forceinline__ double gsum(double v) {
+    if constexpr (G > 1) {
+#pragma unroll
+        for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
+    }
+    return v;
+}
+
+// one tangent evaluation; td selects the direction (0 r, 1 pr, 2 pphi, 3 pphi at pr = 0)
+__device__ __forceinline__ Dual ham_tangent(const HamCoef& hc, double r, double pr, double L, int td) {
+    const Dual R(r, td == 0 ? 1.0 : 0.0);
+    const Dual PR(td == 3 ? 0.0 : pr, td == 1 ? 1.0 : 0.0);
+    const Dual LL(L, td >= 2 ? 1.0 : 0.0);
+    return ham_eval<Dual>(hc, R, PR, LL);
+}
+
+// Everything the flux and the RHS need at one state.
+struct HamPack {
+    double H, dHdr, dHdpr, dHdL, omc;
+};
+
+template <int G>
+__device__ __forceinline__ HamPack ham_pack(const HamCoef& hc, double r, double pr, double L, int gl) {
+    HamPack p;
+    if constexpr (G >= 4) {
+        const Dual d = ham_tangent(hc, r, pr, L, gl & 3);
+        p.H = gshfl<G>(d.v, 0);
+        p.dHdr = gshfl<G>(d.d, 0);
+        p.dHdpr = gshfl<G>(d.d, 1);
+        p.dHdL = gshfl<G>(d.d, 2);
+        p.omc = gshfl<G>(d.d, 3);
+    } else {
+        double dd[4];
+        double hv = 0.0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            // with G = 2 the lanes split the four tangents and exchange
+            if ((t % G) == (gl % G)) {
+                const Dual d = ham_tangent(hc, r, pr, L, t);
+                dd[t] = d.d;
+                if (t == 0) hv = d.v;
+            } else {
+                dd[t] = 0.0;
+            }
+        }
+        if constexpr (G == 1) {
+            p.H = hv; p.dHdr = dd[0]; p.dHdpr = dd[1]; p.dHdL = dd[2]; p.omc = dd[3];
+        } else {
+            p.H = gshfl<G>(hv, 0);
+            p.dHdr = gshfl<G>(dd[0], 0);
+            p.dHdpr = gshfl<G>(dd[1], 1);
+            p.dHdL = gshfl<G>(dd[2], 0);
+            p.omc = gshfl<G>(dd[3], 1);
+        }
+    }
+    return p;
+}
+
+// Flux F (before the 1/nu and 1/Omega factors of RR_force), src/EOB.cu:2506-2560,
+// with the modes split over the group.  omega is the Omega the flux uses.
+template <int G>
+__device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, const BinData& bd, double r, double L, double Hreal,
+                                             double omega, double omc, int gl) {
+    const double v = cbrt(omega);
+    const double logv = log(v);
+    const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
+    const double vPhi = r * cbrt(vPhiIn) * omega;
+    const double vPhi2 = vPhi * vPhi;
+    const double Heff = (Hreal * Hreal - 1.0) / (2.0 * bd.eta_f) + 1.0;
+    const double E1 = exp(-4.0 * kPi * Hreal * omega);
+    double part = 0.0;
+    for (int i = gl; i < kNumModes; i += G) {
+        part += flux_mode(mc[i], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+    }
+    return gsum<G>(part) / kPi / 8.0;
+}
+
+// k = [dH/dpr, dH/dpphi, -dH/dr + Fr, Fphi] (src/EOB.cu:3047-3144), valid on every lane of the group.
+template <int G>
+__device__ __forceinline__ void group_rhs(const BinData& bd, const ModeCoef* __restrict__ mc, double r, double pr, double L, int gl,
+                                          double k[4]) {
+    const HamPack p = ham_pack<G>(bd.h, r, pr, L, gl);
+    const double Hreal = bd.nu * p.H;
+    const double omega = p.dHdL;   // the ODE overwrites omega with dH/dpphi before RR_force
+    double flux = group_flux<G>(mc, bd, r, L, Hreal, omega, p.omc, gl);
+    flux /= bd.nu;
+    const double f_over_om = flux / omega;
+    k[0] = p.dHdpr;
+    k[1] = p.dHdL;
+    k[2] = -p.dHdr + (-pr / L * f_over_om);
+    k[3] = -0.0 + (-f_over_om);
+}
+
+// ---------------------------------------------------------------- (2,2) mode
+struct Cplx {
+    double re, im;
+};
+EOB_HD Cplx cmul(Cplx a, Cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+EOB_HD Cplx cexp_(double re, double im) {
+    const double e = exp(re);
+    return {e * cos(im), e * sin(im)};
+}
+EOB_HD Cplx cdiv(Cplx a, Cplx b) {
+    const double d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+EOB_HD Cplx clog_(Cplx z) { return {log(hypot(z.re, z.im)), atan2(z.im, z.re)}; }
+
+// Gamma(z) for Re z > 0 by shifting to Re z >= 10 and the Stirling series of
+// ln Gamma (Bernoulli terms through B_16); only exp(ln Gamma) is used, so the
+// branch of the logarithm is immaterial.
+EOB_HD Cplx cgamma(Cplx z) {
+    Cplx prod = {1.0, 0.0};
+    while (z.re < 10.0) {
+        prod = cmul(prod, z);
+        z.re += 1.0;
+    }
+    const Cplx iz = cdiv({1.0, 0.0}, z);
+    const Cplx iz2 = cmul(iz, iz);
+    const double b[8] = {1.0 / 12, -1.0 / 360, 1.0 / 1260, -1.0 / 1680, 1.0 / 1188, -691.0 / 360360, 1.0 / 156, -3617.0 / 122400};
+    Cplx ser = {b[7], 0.0};
+    for (int k = 6; k >= 0; --k) {
+        ser = cmul(ser, iz2);
+        ser.re += b[k];
+    }
+    ser = cmul(ser, iz);
+    const Cplx lz = clog_(z);
+    const Cplx t = cmul({z.re - 0.5, z.im}, lz);
+    const double lre = t.re - z.re + 0.9189385332046728 + ser.re;   // + ln(2 pi)/2
+    const double lim = t.im - z.im + ser.im;
+    const Cplx g = cexp_(lre, lim);
+    return cdiv(g, prod);
+}
+
+// h_22 on one knot, src/EOB.cu:1236-1613 for (l, m) = (2, 2).
+// v = Omega^(1/3), Hreal = nu H, vPhiIn = 1/(omega_circ^2 r^3); mc22 is flux mode 1.
+EOB_HD Cplx hlm22_knot(const ModeCoef& mc22, const WaveCoef& wc, double eta, double r, double phi, double v, double Hreal,
+                       double vPhiIn) {
+    if (eta > 0.25 && eta < 0.25 + 1e-4) eta = 0.25;
+    const double v2 = v * v;
+    const double Omega = v2 * v;
+    const double vh3 = Hreal * Omega;
+    const double vh = cbrt(vh3);
+    const double el = kEulerGamma + mc22.log2m + log(v);
+    double vPhi = r * cbrt(vPhiIn);
+    vPhi *= Omega;
+    const double vPhi2 = vPhi * vPhi;
+    // Newtonian multipole prefix * vPhi2 * Y_2^{-2}(pi/2, phi), Y_2^{-2} = sqrt(15/(32 pi)) e^{-2 i phi}
+    const double yamp = 0.3862742020231896;   // sqrt(15 / (32 pi))
+    const Cplx ylm = {yamp * cos(-2.0 * phi), yamp * sin(-2.0 * phi)};
+    Cplx hN = cmul({wc.pf22_re, wc.pf22_im}, ylm);
+    hN.re *= vPhi2; hN.im *= vPhi2;
+    const double Slm = (Hreal * Hreal - 1.0) / (2.0 * eta) + 1.0;
+    const double k = 2.0 * Omega;
+    const double hk = Hreal * k;
+    const Cplx G = cgamma({3.0, -2.0 * hk});
+    // exp(pi hk + 2 i hk ln(4k/sqrt(e))) * Gamma(3 - 2 i hk) / 2!
+    Cplx T = cmul(cexp_(kPi * hk, 2.0 * hk * log(4.0 * k / 1.6487212707001282)), G);
+    T.re *= 0.5; T.im *= 0.5;
+    const double delta = vh3 * (wc.delta22vh3 + vh3 * (wc.delta22vh6 + vh * vh * (wc.delta22vh9 * vh))) + wc.delta22v5 * v * v2 * v2
+                         + wc.delta22v6 * v2 * v2 * v2 + wc.delta22v8 * v2 * v2 * v2 * v2;
+    double acc = mc22.c[10];
+    acc = acc * v;
+    acc = acc * v + (mc22.c[9] + mc22.cl10 * el);
+    acc = acc * v + mc22.c[8];
+    acc = acc * v + (mc22.c[7] + mc22.cl8 * el);
+    acc = acc * v + (mc22.c[6] + mc22.cl7 * el);
+    acc = acc * v + (mc22.c[5] + mc22.cl6 * el);
+    acc = acc * v + mc22.c[4];
+    acc = acc * v + mc22.c[3];
+    acc = acc * v + mc22.c[2];
+    acc = acc * v + mc22.c[1];
+    acc = acc * v + mc22.c[0];
+    const double rho = 1.0 + acc * v;
+    const double amp = Slm * rho * rho;
+    Cplx h = cmul(T, {cos(delta), sin(delta)});
+    h.re *= amp; h.im *= amp;
+    return cmul(h, hN);
+}
+
+}  // namespace eob

@@ -1,0 +1,834 @@
+// HIP kernels and C-ABI entry points for the EOB Hamiltonian, flux, ODE
+// right-hand side, DOP853 stepper and the batched integrators.
+//
+// Entry points mirror the reference's native wraps (see include/eobhip.h for the
+// file:line of each); the batched integrators are this project's design.
+#include <hip/hip_runtime.h>
+#include <cstring>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "eobhip_common.hpp"
+#include "eob_integrator.hpp"
+
+using namespace eob;
+
+namespace eobhip_detail {
+static thread_local char g_err[512] = "";
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+}  // namespace eobhip_detail
+using namespace eobhip_detail;
+
+struct eobhip_seobnr {
+    ModeCoef* modes = nullptr;   // [numSys][35], prefix-independent (N2 set per call)
+    WaveCoef* wave = nullptr;    // [numSys]
+    const double* prefixes_hlms = nullptr;
+    int numSys = 0;
+    bool allocated = false;
+};
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ HamCoef ham_from_add(const double* add, int n, int i) {
+    HamCoef c;
+    ham_setup(c, add[0 * n + i], add[1 * n + i], add[2 * n + i], add[3 * n + i], add[5 * n + i], add[6 * n + i], add[7 * n + i],
+              add[8 * n + i]);
+    return c;
+}
+
+__device__ __forceinline__ double nu_of(double m1, double m2) {
+    const double M = m1 + m2;
+    return (m1 * m2 / M) / M;
+}
+
+// ------------------------------------------------------------- Hamiltonian
+__global__ __launch_bounds__(kBlock) void k_evaluate_H(double* out, const double* r, const double* pr, const double* pphi,
+                                                       const double* m1, const double* m2, const double* c1, const double* c2,
+                                                       double K, double d5, double dSO, double dSS, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        HamCoef c;
+        ham_setup(c, m1[i], m2[i], c1[i], c2[i], K, d5, dSO, dSS);
+        out[i] = ham_eval<double>(c, r[i], pr[i], pphi[i]);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grad_H(const double* arg, double* g, const double* add, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const double r = arg[i], pr = arg[2 * n + i], L = arg[3 * n + i];
+        g[0 * n + i] = ham_tangent(c, r, pr, L, 0).d;
+        g[1 * n + i] = 0.0;
+        g[2 * n + i] = ham_tangent(c, r, pr, L, 1).d;
+        g[3 * n + i] = ham_tangent(c, r, pr, L, 2).d;
+    }
+}
+
+__device__ __forceinline__ double ham_second(const HamCoef& c, double r, double pr, double L, int a, int b) {
+    // a, b in {0: r, 2: pr, 3: pphi}
+    const HDual R(r, a == 0 ? 1.0 : 0.0, b == 0 ? 1.0 : 0.0, 0.0);
+    const HDual PR(pr, a == 2 ? 1.0 : 0.0, b == 2 ? 1.0 : 0.0, 0.0);
+    const HDual LL(L, a == 3 ? 1.0 : 0.0, b == 3 ? 1.0 : 0.0, 0.0);
+    return ham_eval<HDual>(c, R, PR, LL).ab;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hessian_H(const double* arg, double* hs, const double* add, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const double r = arg[i], pr = arg[2 * n + i], L = arg[3 * n + i];
+        for (int q = 0; q < 16; ++q) hs[q * n + i] = 0.0;
+        const int v[3] = {0, 2, 3};
+        for (int ia = 0; ia < 3; ++ia)
+            for (int ib = ia; ib < 3; ++ib) {
+                const double val = ham_second(c, r, pr, L, v[ia], v[ib]);
+                hs[(v[ia] * 4 + v[ib]) * n + i] = val;
+                hs[(v[ib] * 4 + v[ia]) * n + i] = val;
+            }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_IC_cons(double* res, const double* x, double* args, const double* add,
+                                                    double* grad, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double r = x[i], L = x[n + i];
+        args[i] = r; args[n + i] = 0.0; args[2 * n + i] = 0.0; args[3 * n + i] = L;
+        const HamCoef c = ham_from_add(add, n, i);
+        const double dHdr = ham_tangent(c, r, 0.0, L, 0).d;
+        const double dHdL = ham_tangent(c, r, 0.0, L, 2).d;
+        grad[i] = dHdr; grad[n + i] = 0.0; grad[2 * n + i] = ham_tangent(c, r, 0.0, L, 1).d; grad[3 * n + i] = dHdL;
+        res[i] = dHdL - add[4 * n + i];
+        res[n + i] = dHdr;
+    }
+}
+
+// ------------------------------------------------- reference-layout RHS (G = 1)
+// Flux with the handle's rho tables and this call's prefixes / h22_calib
+// (src/EOB.cu:2506-2560); hm points at the 35 modes of the binary.
+__device__ double flux_dropin(const ModeCoef* hm, const double* add, int n, int i, double eta_f, double r, double L, double Hreal,
+                              double omega, double omc) {
+    const double v = cbrt(omega), logv = log(v);
+    const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
+    const double vPhi = r * cbrt(vPhiIn) * omega;
+    const double vPhi2 = vPhi * vPhi;
+    const double Heff = (Hreal * Hreal - 1.0) / (2.0 * eta_f) + 1.0;
+    const double E1 = exp(-4.0 * kPi * Hreal * omega);
+    const double h22c = add[9 * n + i];
+    double sum = 0.0;
+    for (int mi = 0; mi < kNumModes; ++mi) {
+        ModeCoef z = hm[mi];
+        const double pre = add[(10 + 2 * mi) * n + i], pim = add[(11 + 2 * mi) * n + i];
+        const double y = ylm_abs_eq(z.l - (z.lpe - z.l), z.m);
+        z.N2 = (pre * pre + pim * pim) * y * y;
+        if (mi == 1) z.c[10] = h22c;
+        int fl = ((z.l + z.m) & 1) ? 0 : 1;
+        if (eta_f == 0.25 && (z.m & 1)) fl |= 2;
+        z.flags = fl;
+        sum += flux_mode(z, v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+    }
+    return sum / kPi / 8.0;
+}
+
+__device__ void rr_force_dropin(const ModeCoef* hm, const double* add, int n, int i, const HamCoef& c, double r, double pr, double L,
+                                double omega, double F[2]) {
+    const double nu = nu_of(add[i], add[n + i]);
+    double eta_f = nu;
+    if (eta_f > 0.25 && eta_f < 0.25 + 1e-4) eta_f = 0.25;
+    const double omc = ham_tangent(c, r, pr, L, 3).d;
+    const double Hreal = nu * ham_eval<double>(c, r, pr, L);
+    double flux = flux_dropin(hm, add, n, i, eta_f, r, L, Hreal, omega, omc);
+    flux /= nu;
+    const double f_over_om = flux / omega;
+    F[0] = -pr / L * f_over_om;
+    F[1] = -f_over_om;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ode_dropin(const double* arg, double* k, const double* add, int n, const ModeCoef* modes,
+                                                       const int* idx) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const double r = arg[i], pr = arg[2 * n + i], L = arg[3 * n + i];
+        const Dual d0 = ham_tangent(c, r, pr, L, 0);
+        const double dHdpr = ham_tangent(c, r, pr, L, 1).d;
+        const double dHdL = ham_tangent(c, r, pr, L, 2).d;
+        double F[2];
+        rr_force_dropin(modes + (size_t)idx[i] * kNumModes, add, n, i, c, r, pr, L, dHdL, F);
+        k[i] = dHdpr;
+        k[n + i] = dHdL;
+        k[2 * n + i] = -d0.d + F[0];
+        k[3 * n + i] = -0.0 + F[1];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_rr_force_dropin(double* force, double* grad, const double* arg, const double* add, int n,
+                                                            const ModeCoef* modes, const int* idx) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const double r = arg[i], pr = arg[2 * n + i], L = arg[3 * n + i];
+        double F[2];
+        rr_force_dropin(modes + (size_t)idx[i] * kNumModes, add, n, i, c, r, pr, L, add[4 * n + i], F);
+        force[i] = F[0];
+        force[n + i] = F[1];
+        // the reference leaves the pr = 0 gradient in grad_out
+        grad[i] = ham_tangent(c, r, 0.0, L, 0).d;
+        grad[n + i] = 0.0;
+        grad[2 * n + i] = ham_tangent(c, r, 0.0, L, 1).d;
+        grad[3 * n + i] = ham_tangent(c, r, 0.0, L, 3).d;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_info(ModeCoef* modes, WaveCoef* wave, const double* m1, const double* m2,
+                                                        const double* eta, const double* a, const double* chiS, const double* chiA,
+                                                        int use_hm, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        flux_setup(modes + (size_t)i * kNumModes, wave[i], m1[i], m2[i], eta[i], a[i], chiS[i], chiA[i], nullptr, 0.0, use_hm);
+    }
+}
+
+// EOBGetSpinFactorizedWaveform_kernel for (2,2): one block per binary, threads over knots
+__global__ __launch_bounds__(kBlock) void k_hlm22_dropin(double* out, const double* r, const double* phi, const double* v,
+                                                         const double* Hreal, const double* eta, const double* vPhiIn,
+                                                         const double* prefix, double h22c, int numSys, int L,
+                                                         const ModeCoef* modes, const WaveCoef* wave, const int* idx) {
+    for (int b = blockIdx.x; b < numSys; b += gridDim.x) {
+        const int hi = idx[b];
+        ModeCoef z = modes[(size_t)hi * kNumModes + 1];
+        z.c[10] = h22c;
+        WaveCoef w = wave[hi];
+        w.pf22_re = prefix[2 * b];
+        w.pf22_im = prefix[2 * b + 1];
+        for (int s = threadIdx.x; s < L; s += blockDim.x) {
+            const size_t q = (size_t)b * L + s;
+            const Cplx h = hlm22_knot(z, w, eta[b], r[q], phi[q], v[q], Hreal[q], vPhiIn[q]);
+            out[2 * q] = h.re;
+            out[2 * q + 1] = h.im;
+        }
+    }
+}
+
+// --------------------------------------------------- reference stepper kernels
+// ak_term = sum_j a_sj k_j accumulated from 0 in stage order; arg = solOld + h ak;
+// xCurrent = x + c_s h.  (src/DOPR853.cu:197-355, 381-516)
+struct StagePtrs {
+    const double* k[11];
+};
+__constant__ double dA[12][12] = DOP_A;
+__global__ __launch_bounds__(kBlock) void k_stage_arg(double* arg, const double* solOld, const double* x, double* xCur, double* ak,
+                                                      const double* h, StagePtrs kp, int stage, int nargs, int n) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const double hh = h[e];
+        xCur[e] = x[e] + dC[stage] * hh;
+        for (int j = 0; j < nargs; ++j) {
+            double acc = 0.0;
+            for (int q = 0; q < stage; ++q) {
+                const double a = dA[stage][q];
+                if (a != 0.0) acc += a * kp.k[q][j * n + e];
+            }
+            ak[j * n + e] = acc;
+            arg[j * n + e] = solOld[j * n + e] + hh * acc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_error(double* err, const double* solOld, double* solNew, const double* h, const double* k1,
+                                                  const double* k2, const double* k3, const double* k6, const double* k7,
+                                                  const double* k8, const double* k9, const double* k10, int n, int nargs,
+                                                  double rtol, double atol) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        double et = 0.0, e2 = 0.0;
+        const double hh = h[e];
+        for (int j = 0; j < nargs; ++j) {
+            const int x = j * n + e;
+            const double temp = kB1 * k1[x] + kB6 * k6[x] + kB7 * k7[x] + kB8 * k8[x] + kB9 * k9[x] + kB10 * k10[x] + kB11 * k2[x] + kB12 * k3[x];
+            const double yo = solOld[x];
+            const double yn = yo + hh * temp;
+            solNew[x] = yn;
+            const double mx = (yo > yn) ? yo : yn;
+            const double sk = 1.0 / (atol + rtol * mx);
+            e2 += pow((temp - kBHH1 * k1[x] - kBHH2 * k9[x] - kBHH3 * k3[x]) * sk, (double)nargs);
+            et += pow((kE1 * k1[x] + kE6 * k6[x] + kE7 * k7[x] + kE8 * k8[x] + kE9 * k9[x] + kE10 * k10[x] + kE11 * k2[x] + kE12 * k3[x]) * sk,
+                      (double)nargs);
+        }
+        double denom = et + 0.01 * e2;
+        denom = (denom < 0.0) ? 1.0 : denom;
+        err[e] = fabs(hh) * et * sqrt(1.0 / ((double)nargs * denom));
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_controller(unsigned char* flag, const double* err, double* errOld, unsigned char* prevRej,
+                                                       double* h, int n) {
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        const double er = err[e];
+        if (er <= 1.0) {
+            double scale;
+            if (er == 0.0) scale = kMaxScale;
+            else {
+                scale = kSafe * pow(er, -kAlpha);   // errOld^beta with beta = 0
+                scale = (scale > kMaxScale) ? kMaxScale : ((scale < kMinScale) ? kMinScale : scale);
+            }
+            if (prevRej[e] && scale > 1.0) scale = 1.0;
+            h[e] = h[e] * scale;
+            double v = er;
+            v = (v < 1e-4) ? 1e-4 : ((v > 1e300) ? 1e300 : v);
+            errOld[e] = v;
+            prevRej[e] = 0;
+            flag[e] = 1;
+        } else {
+            prevRej[e] = 1;
+            flag[e] = 0;
+            double t = kSafe * pow(er, -kAlpha);
+            t = (t < kMinScale) ? kMinScale : ((t > 1e300) ? 1e300 : t);
+            h[e] = h[e] * t;
+        }
+    }
+}
+
+// ------------------------------------------------------------ batch pipeline
+__global__ __launch_bounds__(kBlock) void k_bin_setup(const double* add, int n, int use_hm, BinData* bd, ModeCoef* modes) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double m1 = add[i], m2 = add[n + i], c1 = add[2 * n + i], c2 = add[3 * n + i];
+        BinData d;
+        ham_setup(d.h, m1, m2, c1, c2, add[5 * n + i], add[6 * n + i], add[7 * n + i], add[8 * n + i]);
+        d.nu = nu_of(m1, m2);
+        double ef = d.nu;
+        if (ef > 0.25 && ef < 0.25 + 1e-4) ef = 0.25;
+        d.eta_f = ef;
+        // setup_class inputs (eob/seobnrv4phm.py:1924-1929)
+        const double eta = m1 * m2 / ((m1 + m2) * (m1 + m2));
+        const double chiS = 0.5 * (c1 + c2), chiA = 0.5 * (c1 - c2);
+        const double a = (1 - 2 * eta) * chiS + (m1 - m2) / (m1 + m2) * chiA;
+        double pf[70];
+        for (int q = 0; q < 70; ++q) pf[q] = add[(10 + q) * n + i];
+        ModeCoef* mc = modes + (size_t)i * kNumModes;
+        flux_setup(mc, d.w, m1, m2, eta, a, chiS, chiA, pf, add[9 * n + i], use_hm);
+        // the flux's mode flags follow the flux eta
+        for (int q = 0; q < kNumModes; ++q) {
+            int fl = ((mc[q].l + mc[q].m) & 1) ? 0 : 1;
+            if (ef == 0.25 && (mc[q].m & 1)) fl |= 2;
+            mc[q].flags = fl;
+        }
+        bd[i] = d;
+    }
+}
+
+// full G = 1 RHS on precomputed tables (initial conditions, knots)
+__device__ __forceinline__ void rhs1(const BinData& bd, const ModeCoef* mc, double r, double pr, double L, double k[4]) {
+    group_rhs<1>(bd, mc, r, pr, L, 0, k);
+}
+
+__device__ double ic_diss(const BinData& bd, const ModeCoef* mc, double r, double pr, double L) {
+    double k[4];
+    rhs1(bd, mc, r, pr, L, k);
+    const double hrr = ham_second(bd.h, r, pr, L, 0, 0);
+    const double hrL = ham_second(bd.h, r, pr, L, 0, 3);
+    const double dLdr = -hrr / hrL;
+    const double rdot = k[3] / dLdr;
+    return rdot - k[0];
+}
+
+__global__ __launch_bounds__(64) void k_initial_conditions(const double* r0, const BinData* bdp, const ModeCoef* modes, int n,
+                                                           double* pphi0, double* pr0, double* omega0, int max_iter, double err) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const BinData bd = bdp[i];
+        const ModeCoef* mc = modes + (size_t)i * kNumModes;
+        const double r = r0[i];
+        double L = sqrt(r);
+        for (int it = 0; it < 100; ++it) {
+            const HDual R(r, 1.0, 0.0, 0.0), PR(0.0), LL(L, 0.0, 1.0, 0.0);
+            const HDual H = ham_eval<HDual>(bd.h, R, PR, LL);
+            const double dL = H.a / H.ab;
+            L -= dL;
+            if (fabs(dL) < 1e-15 * fabs(L)) break;
+        }
+        omega0[i] = ham_tangent(bd.h, r, 0.0, L, 2).d;
+        double a = -3e-2, b = 0.0;
+        double fa = ic_diss(bd, mc, r, a, L), fb = ic_diss(bd, mc, r, b, L);
+        double c = 0.0;
+        for (int it = 0; it < max_iter; ++it) {
+            c = (a * fb - b * fa) / (fb - fa);
+            const double fc = ic_diss(bd, mc, r, c, L);
+            if ((fb < 0.0 && fc < 0.0) || (fb >= 0.0 && fc >= 0.0)) { b = c; fb = fc; }
+            else { a = c; fa = fc; }
+            if (fabs(fc) < err) break;
+        }
+        pphi0[i] = L;
+        pr0[i] = c;
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) {
+    const int gl = threadIdx.x % G;
+    GState<G> s;
+    s.pass = 0;
+    s.b = 0;
+    bool exhausted = false;
+    while (true) {
+        if (!exhausted && s.pass == 0) {
+            int b = 0;
+            if (gl == 0) b = atomicAdd(P.queue, 1);
+            b = gshfl_i<G>(b, 0);
+            if (b >= P.nbin) exhausted = true;
+            else begin_binary<G>(P, s, b, gl);
+        }
+        if (__all(exhausted)) break;
+        if (!exhausted) {
+            bool need_new;
+            iterate<G>(P, s, gl, need_new);
+        }
+    }
+}
+
+// plain mode: state kept in global memory between launches
+struct PlainState {
+    int *b, *pass, *step_num, *sn1, *code, *flags;   // flags: attempted | just_acc<<1 | stop_flag<<2 | prev_reject<<3
+    double *x, *h, *err_old, *h_init, *om, *y;       // om [4][n], y [4][n]
+    unsigned long long* att;
+};
+
+template <int G>
+__device__ __forceinline__ void load_state(const PlainState& S, GState<G>& s, int b, int gl, int n) {
+    s.b = b;
+    s.pass = S.pass[b]; s.step_num = S.step_num[b]; s.sn1 = S.sn1[b]; s.code = S.code[b];
+    const int f = S.flags[b];
+    s.attempted = f & 1; s.just_acc = (f >> 1) & 1; s.stop_flag = (f >> 2) & 1; s.prev_reject = (f >> 3) & 1;
+    s.fail = (f >> 4) & 3;
+    s.x = S.x[b]; s.h = S.h[b]; s.err_old = S.err_old[b]; s.h_init = S.h_init[b];
+    s.om0 = S.om[b]; s.om1 = S.om[n + b]; s.om2 = S.om[2 * n + b]; s.om3 = S.om[3 * n + b];
+    s.att = S.att[b];
+#pragma unroll
+    for (int q = 0; q < GState<G>::NC; ++q) {
+        const int c = comp_of<G>(gl, q);
+        s.y[q] = (c < 4) ? S.y[(size_t)c * n + b] : 0.0;
+    }
+}
+
+template <int G>
+__device__ __forceinline__ void store_state(const PlainState& S, const GState<G>& s, int gl, int n) {
+    const int b = s.b;
+    if (gl == 0) {
+        S.pass[b] = s.pass; S.step_num[b] = s.step_num; S.sn1[b] = s.sn1; S.code[b] = s.code;
+        S.flags[b] = s.attempted | (s.just_acc << 1) | (s.stop_flag << 2) | (s.prev_reject << 3) | (s.fail << 4);
+        S.x[b] = s.x; S.h[b] = s.h; S.err_old[b] = s.err_old; S.h_init[b] = s.h_init;
+        S.om[b] = s.om0; S.om[n + b] = s.om1; S.om[2 * n + b] = s.om2; S.om[3 * n + b] = s.om3;
+        S.att[b] = s.att;
+    }
+#pragma unroll
+    for (int q = 0; q < GState<G>::NC; ++q) {
+        const int c = comp_of<G>(gl, q);
+        if (c < 4) S.y[(size_t)c * n + b] = s.y[q];
+    }
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_plain_init(IntegParams P, PlainState S, int* active) {
+    const int gi = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int gl = threadIdx.x % G;
+    if (gi >= P.nbin) return;
+    GState<G> s;
+    begin_binary<G>(P, s, gi, gl);
+    store_state<G>(S, s, gl, P.nbin);
+    if (gl == 0) active[gi] = gi;
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_plain_step(IntegParams P, PlainState S, const int* active_in, int n_active,
+                                                       int* active_out, int* n_out) {
+    const int gi = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int gl = threadIdx.x % G;
+    if (gi >= n_active) return;
+    const int b = active_in[gi];
+    GState<G> s;
+    load_state<G>(S, s, b, gl, P.nbin);
+    bool need_new;
+    iterate<G>(P, s, gl, need_new);
+    store_state<G>(S, s, gl, P.nbin);
+    if (gl == 0 && (s.pass == 1 || s.pass == 2)) {
+        const int slot = atomicAdd(n_out, 1);
+        active_out[slot] = b;
+    }
+}
+
+// (2,2) mode on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221)
+__global__ __launch_bounds__(kBlock) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
+                                                        const ModeCoef* modes, int n, int max_step, double* knots) {
+    const int b = blockIdx.y;
+    if (b >= n) return;
+    const int len = num_steps[b];
+    const BinData& bd = bdp[b];
+    const ModeCoef& mc22 = modes[(size_t)b * kNumModes + 1];
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < len; s += gridDim.x * blockDim.x) {
+        const double* row = dense + ((size_t)b * max_step + s) * 5;
+        const double r = row[1], phi = row[2], pr = row[3], L = row[4];
+        const Dual dfull = ham_tangent(bd.h, r, pr, L, 2);
+        const double omega = dfull.d;
+        const double omc = ham_tangent(bd.h, r, pr, L, 3).d;
+        const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
+        const double Hreal = bd.nu * dfull.v;
+        const double v = cbrt(omega);
+        const Cplx h = hlm22_knot(mc22, bd.w, bd.nu, r, phi, v, Hreal, vPhiIn);
+        // h e^{2 i phi}: the slowly varying part the reference splines (eob/seobnrv4phm.py:2338)
+        const double cp = cos(2.0 * phi), sp = sin(2.0 * phi);
+        double* o = knots + ((size_t)b * max_step + s) * 3;
+        o[0] = phi;
+        o[1] = h.re * cp - h.im * sp;
+        o[2] = h.re * sp + h.im * cp;
+    }
+}
+
+template <int G>
+int launch_persistent(const IntegParams& P, int block, int grid_groups, hipStream_t st) {
+    const int groups_per_block = block / G;
+    int nblk = (grid_groups + groups_per_block - 1) / groups_per_block;
+    if (nblk < 1) nblk = 1;
+    hipLaunchKernelGGL(k_integrate_persistent<G>, dim3(nblk), dim3(block), 0, st, P);
+    return check_launch("k_integrate_persistent");
+}
+
+template <int G>
+int run_plain(const IntegParams& P, PlainState S, int* act0, int* act1, int* cnt, int* host_cnt, int block, hipStream_t st,
+              long long* iters) {
+    const int gpb = block / G;
+    hipLaunchKernelGGL(k_plain_init<G>, dim3(blocks_for(P.nbin, gpb)), dim3(block), 0, st, P, S, act0);
+    int rc = check_launch("k_plain_init");
+    if (rc) return rc;
+    int n_active = P.nbin;
+    long long it = 0;
+    while (n_active > 0) {
+        rc = hip_status(hipMemsetAsync(cnt, 0, sizeof(int), st), "memset");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_plain_step<G>, dim3(blocks_for(n_active, gpb)), dim3(block), 0, st, P, S, act0, n_active, act1, cnt);
+        rc = check_launch("k_plain_step");
+        if (rc) return rc;
+        rc = hip_status(hipMemcpyAsync(host_cnt, cnt, sizeof(int), hipMemcpyDeviceToHost, st), "count copy");
+        if (rc) return rc;
+        rc = hip_status(hipStreamSynchronize(st), "sync");
+        if (rc) return rc;
+        n_active = *host_cnt;
+        int* t = act0; act0 = act1; act1 = t;
+        ++it;
+    }
+    if (iters) *iters = it;
+    return EOBHIP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* eobhip_last_error(void) { return g_err; }
+int eobhip_version(void) { return 1; }
+
+int eobhip_dop853_tableau(double* out) {
+    EOBHIP_REQUIRE(out, "null output");
+    constexpr double A[12][12] = DOP_A;
+    const double c[12] = {0.0, 0.05260015195876773, 0.0789002279381516, 0.1183503419072274, 0.2816496580927726, 0.3333333333333333,
+                          0.25, 0.3076923076923077, 0.6512820512820513, 0.6, 0.8571428571428571, 1.0};
+    const double b[12] = {kB1, 0, 0, 0, 0, kB6, kB7, kB8, kB9, kB10, kB11, kB12};
+    const double e[12] = {kE1, 0, 0, 0, 0, kE6, kE7, kE8, kE9, kE10, kE11, kE12};
+    int o = 0;
+    for (int i = 0; i < 12; ++i) out[o++] = c[i];
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) out[o++] = A[i][j];
+    for (int i = 0; i < 12; ++i) out[o++] = b[i];
+    for (int i = 0; i < 12; ++i) out[o++] = e[i];
+    out[o++] = kBHH1; out[o++] = kBHH2; out[o++] = kBHH3;
+    return EOBHIP_OK;
+}
+
+int eobhip_evaluate_Ham_align_AD(double* out, const double* r, const double* phi, const double* pr, const double* pphi,
+                                 const double* m_1, const double* m_2, const double* chi_1, const double* chi_2, double K, double d5,
+                                 double dSO, double dSS, int numSys, void* stream) {
+    (void)phi;
+    EOBHIP_REQUIRE(numSys >= 0, "numSys < 0");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_evaluate_H, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), out, r, pr, pphi, m_1, m_2,
+                       chi_1, chi_2, K, d5, dSO, dSS, numSys);
+    return check_launch("evaluate_Ham_align_AD");
+}
+
+int eobhip_grad_Ham_align_AD(const double* arg, double* grad_out, const double* add, int numSys, void* stream) {
+    EOBHIP_REQUIRE(numSys >= 0, "numSys < 0");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_grad_H, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), arg, grad_out, add, numSys);
+    return check_launch("grad_Ham_align_AD");
+}
+
+int eobhip_hessian_Ham_align_AD(const double* arg, double* hess_out, const double* add, int numSys, void* stream) {
+    EOBHIP_REQUIRE(numSys >= 0, "numSys < 0");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_hessian_H, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), arg, hess_out, add, numSys);
+    return check_launch("hessian_Ham_align_AD");
+}
+
+int eobhip_IC_cons(double* res, const double* x, double* args, const double* add, double* grad_out, int numSys, void* stream) {
+    EOBHIP_REQUIRE(numSys >= 0, "numSys < 0");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_IC_cons, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), res, x, args, add, grad_out,
+                       numSys);
+    return check_launch("IC_cons");
+}
+
+int eobhip_seobnr_create(eobhip_seobnr_t* out) {
+    EOBHIP_REQUIRE(out, "null handle pointer");
+    *out = new eobhip_seobnr();
+    return EOBHIP_OK;
+}
+
+int eobhip_seobnr_deallocate_information(eobhip_seobnr_t h) {
+    EOBHIP_REQUIRE(h, "null handle");
+    if (h->allocated) {
+        hipFree(h->modes);
+        hipFree(h->wave);
+        h->modes = nullptr;
+        h->wave = nullptr;
+        h->allocated = false;
+        h->numSys = 0;
+    }
+    return EOBHIP_OK;
+}
+
+int eobhip_seobnr_destroy(eobhip_seobnr_t h) {
+    if (!h) return EOBHIP_OK;
+    eobhip_seobnr_deallocate_information(h);
+    delete h;
+    return EOBHIP_OK;
+}
+
+int eobhip_seobnr_update_information(eobhip_seobnr_t h, const double* m1, const double* m2, const double* eta, const double* tplspin,
+                                     const double* chi_S, const double* chi_A, int use_hm, int numSys,
+                                     const double* newtonian_prefixes_hlms, void* stream) {
+    EOBHIP_REQUIRE(h, "null handle");
+    EOBHIP_REQUIRE(!h->allocated, "update_information: information already allocated (call deallocate_information first)");
+    EOBHIP_REQUIRE(numSys > 0, "numSys must be positive");
+    int rc = hip_status(hipMalloc(&h->modes, sizeof(ModeCoef) * (size_t)numSys * kNumModes), "hipMalloc modes");
+    if (rc) return rc;
+    rc = hip_status(hipMalloc(&h->wave, sizeof(WaveCoef) * (size_t)numSys), "hipMalloc wave");
+    if (rc) { hipFree(h->modes); h->modes = nullptr; return rc; }
+    h->numSys = numSys;
+    h->prefixes_hlms = newtonian_prefixes_hlms;
+    h->allocated = true;
+    hipLaunchKernelGGL(k_update_info, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), h->modes, h->wave, m1, m2,
+                       eta, tplspin, chi_S, chi_A, use_hm, numSys);
+    return check_launch("update_information");
+}
+
+int eobhip_seobnr_RR_force(eobhip_seobnr_t h, double* force_out, double* grad_out, const double* args, const double* add, int numSys,
+                           const int* idx, void* stream) {
+    EOBHIP_REQUIRE(h && h->allocated, "RR_force: update_information has not been called");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_rr_force_dropin, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), force_out, grad_out, args,
+                       add, numSys, h->modes, idx);
+    return check_launch("RR_force");
+}
+
+int eobhip_seobnr_ODE_Ham_align_AD(eobhip_seobnr_t h, const double* x, const double* arg, double* k, const double* add, int numSys,
+                                   const int* idx, void* stream) {
+    (void)x;
+    EOBHIP_REQUIRE(h && h->allocated, "ODE_Ham_align_AD: update_information has not been called");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_ode_dropin, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), arg, k, add, numSys, h->modes,
+                       idx);
+    return check_launch("ODE_Ham_align_AD");
+}
+
+int eobhip_seobnr_EOBGetSpinFactorizedWaveform(eobhip_seobnr_t h, double* out, const double* r, const double* phi, const double* pr,
+                                               const double* pphi, const double* v, const double* Hreal, const double* eta,
+                                               const double* vPhiInput, int l, int m, const double* newtonian_prefix, double h22_calib,
+                                               int numSys, int traj_length, const int* idx, void* stream) {
+    (void)pr; (void)pphi;
+    EOBHIP_REQUIRE(h && h->allocated, "EOBGetSpinFactorizedWaveform: update_information has not been called");
+    if (!(l == 2 && m == 2)) {
+        set_error("EOBGetSpinFactorizedWaveform: only (l, m) = (2, 2) is implemented, got (%d, %d)", l, m);
+        return EOBHIP_EUNSUPPORTED;
+    }
+    if (numSys == 0 || traj_length == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_hlm22_dropin, dim3(numSys), dim3(kBlock), 0, as_stream(stream), out, r, phi, v, Hreal, eta, vPhiInput,
+                       newtonian_prefix, h22_calib, numSys, traj_length, h->modes, h->wave, idx);
+    return check_launch("EOBGetSpinFactorizedWaveform");
+}
+
+int eobhip_dormandPrinceSteps(double* x, double* xCurrent, double* solOld, double* h, double* arg, double* add, double* k1, double* k2,
+                              double* k3, double* k4, double* k5, double* k6, double* k7, double* k8, double* k9, double* k10,
+                              double* ak, int nargs, int numEq, int num_add_args, eobhip_seobnr_t ode, const int* idx, void* stream) {
+    (void)num_add_args;
+    EOBHIP_REQUIRE(ode && ode->allocated, "dormandPrinceSteps: ode handle without information");
+    EOBHIP_REQUIRE(nargs == 4, "dormandPrinceSteps: nargs must be 4");
+    if (numEq == 0) return EOBHIP_OK;
+    hipStream_t st = as_stream(stream);
+    // stage s result buffers in reference order: stages 1..10 -> k1..k10, 11 -> k2, 12 -> k3
+    double* outs[12] = {k1, k2, k3, k4, k5, k6, k7, k8, k9, k10, k2, k3};
+    StagePtrs kp;
+    const double* ins[11] = {k1, k2, k3, k4, k5, k6, k7, k8, k9, k10, k2};
+    for (int q = 0; q < 11; ++q) kp.k[q] = ins[q];
+    const unsigned nb = blocks_for(numEq, kBlock);
+    for (int s = 0; s < 12; ++s) {
+        // stage 12 reads stage 11 from k2 (ins[10]) and stage 2 never enters it (a_12,2 = 0)
+        hipLaunchKernelGGL(k_stage_arg, dim3(nb), dim3(kBlock), 0, st, arg, solOld, x, xCurrent, ak, h, kp, s, nargs, numEq);
+        int rc = check_launch("stage args");
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_ode_dropin, dim3(nb), dim3(kBlock), 0, st, arg, outs[s], add, numEq, ode->modes, idx);
+        rc = check_launch("stage rhs");
+        if (rc) return rc;
+    }
+    return EOBHIP_OK;
+}
+
+int eobhip_error_tol(double* err, const double* solOld, double* solNew, const double* h, const double* k1, const double* k2,
+                     const double* k3, const double* k6, const double* k7, const double* k8, const double* k9, const double* k10, int numEq,
+                     int nargs, double rtol, double atol, void* stream) {
+    if (numEq == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_error, dim3(blocks_for(numEq, kBlock)), dim3(kBlock), 0, as_stream(stream), err, solOld, solNew, h, k1, k2, k3,
+                       k6, k7, k8, k9, k10, numEq, nargs, rtol, atol);
+    return check_launch("error");
+}
+
+int eobhip_error(double* err, const double* solOld, double* solNew, const double* h, const double* k1, const double* k2, const double* k3,
+                 const double* k6, const double* k7, const double* k8, const double* k9, const double* k10, int numEq, int nargs,
+                 void* stream) {
+    return eobhip_error_tol(err, solOld, solNew, h, k1, k2, k3, k6, k7, k8, k9, k10, numEq, nargs, 1e-12, 1e-12, stream);
+}
+
+int eobhip_controllerSuccess(unsigned char* flag, const double* err, double* errOld, unsigned char* prevRej, double* h, const double* x,
+                             int numEq, int nargs, void* stream) {
+    (void)x; (void)nargs;
+    if (numEq == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_controller, dim3(blocks_for(numEq, kBlock)), dim3(kBlock), 0, as_stream(stream), flag, err, errOld, prevRej, h,
+                       numEq);
+    return check_launch("controllerSuccess");
+}
+
+size_t eobhip_bindata_bytes(void) { return sizeof(BinData); }
+size_t eobhip_modes_bytes(void) { return sizeof(ModeCoef) * kNumModes; }
+
+int eobhip_bin_setup(const double* add, int n, int use_hm, void* bindata, void* modes, void* stream) {
+    EOBHIP_REQUIRE(n >= 0, "n < 0");
+    if (n == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_bin_setup, dim3(blocks_for(n, 64)), dim3(64), 0, as_stream(stream), add, n, use_hm,
+                       static_cast<BinData*>(bindata), static_cast<ModeCoef*>(modes));
+    return check_launch("bin_setup");
+}
+
+int eobhip_initial_conditions(const double* r0, const void* bindata, const void* modes, int n, double* pphi0, double* pr0,
+                              double* omega0, int max_iter, double err, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && max_iter >= 1, "bad arguments");
+    if (n == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_initial_conditions, dim3(blocks_for(n, 64)), dim3(64), 0, as_stream(stream), r0,
+                       static_cast<const BinData*>(bindata), static_cast<const ModeCoef*>(modes), n, pphi0, pr0, omega0, max_iter, err);
+    return check_launch("initial_conditions");
+}
+
+size_t eobhip_integrate_scratch_bytes(int n) {
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    // queue/count ints + plain-mode state: 6 int arrays, 4 + 4 + 4 double rows, att, 2 active lists
+    // 16 sub-arrays, each padded to a 256-byte boundary
+    return 256 + nn * (6 * sizeof(int) + 12 * sizeof(double) + sizeof(unsigned long long) + 2 * sizeof(int)) + 16 * 256;
+}
+
+int eobhip_integrate(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol, double atol,
+                     double hInit, double step_back, double fine_step, int two_pass, int fix_step, int mode, int group, int block,
+                     int grid_groups, double* dense, int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts,
+                     void* scratch, long long* iterations_out, void* stream) {
+    EOBHIP_REQUIRE(n >= 0, "n < 0");
+    EOBHIP_REQUIRE(max_step >= 2, "max_step must be >= 2");
+    EOBHIP_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (persistent) or 1 (plain)");
+    EOBHIP_REQUIRE(group == 1 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
+                   "group must be one of 1, 4, 8, 16, 32, 64");
+    if (block <= 0) block = kBlock;
+    EOBHIP_REQUIRE(block % 64 == 0 && block <= kBlock && block >= group, "block must be a multiple of 64, <= 256 and >= group");
+    EOBHIP_REQUIRE(scratch != nullptr, "scratch is required");
+    if (iterations_out) *iterations_out = 0;
+    if (n == 0) return EOBHIP_OK;
+    hipStream_t st = as_stream(stream);
+
+    IntegParams P;
+    P.cond = condBound;
+    P.bd = static_cast<const BinData*>(bindata);
+    P.mc = static_cast<const ModeCoef*>(modes);
+    P.dense = dense;
+    P.num_steps = num_steps;
+    P.stop_info = stop_info;
+    P.pass1_steps = pass1_steps;
+    P.attempts = attempts;
+    P.nbin = n;
+    P.max_step = max_step;
+    P.rtol = rtol; P.atol = atol; P.hInit = hInit; P.step_back = step_back; P.fine_step = fine_step; P.tmax = 1e13;
+    P.two_pass = two_pass ? 1 : 0;
+    P.fix_step = fix_step ? 1 : 0;
+    P.max_attempts = 16ull * (unsigned long long)max_step + 4096ull;
+
+    // carve scratch
+    char* base = static_cast<char*>(scratch);
+    size_t off = 0;
+    auto take = [&](size_t bytes) { void* p = base + off; off += (bytes + 255) & ~size_t(255); return p; };
+    int* queue = static_cast<int*>(take(sizeof(int) * 2));
+    P.queue = queue;
+    int rc = hip_status(hipMemsetAsync(queue, 0, sizeof(int) * 2, st), "memset queue");
+    if (rc) return rc;
+
+    if (mode == 0) {
+        int gg = grid_groups > 0 ? grid_groups : n;
+        switch (group) {
+            case 1: return launch_persistent<1>(P, block, gg, st);
+            case 4: return launch_persistent<4>(P, block, gg, st);
+            case 8: return launch_persistent<8>(P, block, gg, st);
+            case 16: return launch_persistent<16>(P, block, gg, st);
+            case 32: return launch_persistent<32>(P, block, gg, st);
+            default: return launch_persistent<64>(P, block, gg, st);
+        }
+    }
+    const size_t nn = (size_t)n;
+    PlainState S;
+    S.b = static_cast<int*>(take(nn * sizeof(int)));
+    S.pass = static_cast<int*>(take(nn * sizeof(int)));
+    S.step_num = static_cast<int*>(take(nn * sizeof(int)));
+    S.sn1 = static_cast<int*>(take(nn * sizeof(int)));
+    S.code = static_cast<int*>(take(nn * sizeof(int)));
+    S.flags = static_cast<int*>(take(nn * sizeof(int)));
+    S.x = static_cast<double*>(take(nn * sizeof(double)));
+    S.h = static_cast<double*>(take(nn * sizeof(double)));
+    S.err_old = static_cast<double*>(take(nn * sizeof(double)));
+    S.h_init = static_cast<double*>(take(nn * sizeof(double)));
+    S.om = static_cast<double*>(take(4 * nn * sizeof(double)));
+    S.y = static_cast<double*>(take(4 * nn * sizeof(double)));
+    S.att = static_cast<unsigned long long*>(take(nn * sizeof(unsigned long long)));
+    int* act0 = static_cast<int*>(take(nn * sizeof(int)));
+    int* act1 = static_cast<int*>(take(nn * sizeof(int)));
+    if (off > eobhip_integrate_scratch_bytes(n)) {
+        set_error("internal: scratch layout exceeds eobhip_integrate_scratch_bytes");
+        return EOBHIP_EINVAL;
+    }
+    int* host_cnt = nullptr;
+    rc = hip_status(hipHostMalloc(&host_cnt, sizeof(int)), "hipHostMalloc");
+    if (rc) return rc;
+    switch (group) {
+        case 1: rc = run_plain<1>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
+        case 4: rc = run_plain<4>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
+        case 8: rc = run_plain<8>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
+        case 16: rc = run_plain<16>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
+        case 32: rc = run_plain<32>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
+        default: rc = run_plain<64>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
+    }
+    hipHostFree(host_cnt);
+    return rc;
+}
+
+int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n, int max_step,
+                       double* knots, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && max_step >= 1, "bad arguments");
+    if (n == 0) return EOBHIP_OK;
+    EOBHIP_REQUIRE(n <= 65535, "n must be <= 65535 (grid y dimension)");
+    const int bx = (max_step + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_hlm22_knots, dim3(bx < 64 ? bx : 64, n), dim3(kBlock), 0, as_stream(stream), dense, num_steps,
+                       static_cast<const BinData*>(bindata), static_cast<const ModeCoef*>(modes), n, max_step, knots);
+    return check_launch("hlm22_knots");
+}
+
+}  // extern "C"

@@ -1,0 +1,400 @@
+// Batched not-a-knot cubic splines, uniform resampling of the (2,2) mode, the
+// real-to-complex FFT of the templates and the frequency-domain likelihood.
+//
+// Reference: interpolate_TD (src/interpolate.cu:11-308), CubicSplineInterpolantTD
+// (eob/seobnrv4phm.py:161-386), compute_full_waveform's resampling of
+// h_lm e^{i m phi} and phi (eob/seobnrv4phm.py:2319-2372), the likelihood
+// all_in_one_likelihood (src/interpolate.cu:553-628) with the host finish in
+// get_ll (eob/seobnrv4phm.py:772-823).
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <map>
+#include <mutex>
+#include <utility>
+
+#include "eobhip_common.hpp"
+
+using namespace eobhip_detail;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr double kPi = 3.141592653589793;
+
+// ----------------------------------------------------------------- tridiagonal
+// Row i of the not-a-knot system in the slopes s (src/interpolate.cu:11-120):
+//   interior: dx2 s_{i-1} + 2 (dx1 + dx2) s_i + dx1 s_{i+1} = 3 (dx2 m1 + dx1 m2)
+//   first/last rows: the not-a-knot end conditions.
+struct Row {
+    double b, ud, dg, ld;
+};
+
+__device__ __forceinline__ Row nak_row(int i, int len, double x0, double x1, double x2, double y0, double y1, double y2) {
+    const double dx1 = x1 - x0, dx2 = x2 - x1, d = x2 - x0;
+    const double s1 = (y1 - y0) / dx1, s2 = (y2 - y1) / dx2;
+    Row r;
+    if (i == len - 1) {
+        r.b = (dx2 * dx2 * s1 + (2 * d + dx2) * dx1 * s2) / d;
+        r.dg = dx1; r.ld = d; r.ud = 0.0;
+    } else if (i == 0) {
+        r.b = ((dx1 + 2 * d) * dx2 * s1 + dx1 * dx1 * s2) / d;
+        r.ud = d; r.ld = 0.0; r.dg = dx2;
+    } else {
+        r.b = 3.0 * (dx2 * s1 + dx1 * s2);
+        r.dg = 2 * (dx1 + dx2); r.ud = dx1; r.ld = dx2;
+    }
+    return r;
+}
+
+// Drop-in layout: t [len][nbin], y/B/diagonals [len][ninterp], interpolant
+// j = sub * nbin + bin.  One thread per interpolant; neighbouring threads walk
+// neighbouring interpolants, so every knot step is one coalesced row access.
+__global__ __launch_bounds__(kBlock) void k_interpolate_TD(const double* __restrict__ t, const double* __restrict__ y, double* B,
+                                                           double* ud, double* dg, double* ld, const int* __restrict__ lengths,
+                                                           int nbin, int ninterp) {
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= ninterp) return;
+    const int b = it % nbin;
+    const int len = lengths[b];
+    if (len < 3) return;
+    // forward: assemble and eliminate in one sweep (Thomas, src/interpolate.cu:161-205)
+    double prev_dg = 0.0, prev_ud = 0.0, prev_b = 0.0;
+    for (int i = 0; i < len; ++i) {
+        int i0, i1, i2;
+        if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
+        else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
+        else { i0 = i - 1; i1 = i; i2 = i + 1; }
+        Row r = nak_row(i, len, t[(size_t)i0 * nbin + b], t[(size_t)i1 * nbin + b], t[(size_t)i2 * nbin + b], y[(size_t)i0 * ninterp + it],
+                        y[(size_t)i1 * ninterp + it], y[(size_t)i2 * ninterp + it]);
+        const size_t o = (size_t)i * ninterp + it;
+        ld[o] = r.ld;
+        if (i > 0) {
+            const double w = r.ld / prev_dg;
+            r.dg = r.dg - w * prev_ud;
+            r.b = r.b - w * prev_b;
+        }
+        ud[o] = r.ud; dg[o] = r.dg; B[o] = r.b;
+        prev_dg = r.dg; prev_ud = r.ud; prev_b = r.b;
+    }
+    // back substitution
+    size_t o = (size_t)(len - 1) * ninterp + it;
+    double snext = B[o] / dg[o];
+    B[o] = snext;
+    for (int i = len - 2; i >= 0; --i) {
+        o = (size_t)i * ninterp + it;
+        snext = (B[o] - ud[o] * snext) / dg[o];
+        B[o] = snext;
+    }
+    // polynomial coefficients into ud (c1), dg (c2), ld (c3) for knots 0..len-2
+    double s_i = B[it];
+    for (int i = 0; i < len - 1; ++i) {
+        const size_t oi = (size_t)i * ninterp + it, op = (size_t)(i + 1) * ninterp + it;
+        const double dt = t[(size_t)(i + 1) * nbin + b] - t[(size_t)i * nbin + b];
+        const double s_p = B[op];
+        const double slope = (y[op] - y[oi]) / dt;
+        const double tt = (s_i + s_p - 2 * slope) / dt;
+        ud[oi] = s_i;
+        dg[oi] = (slope - s_i) / dt - tt;
+        ld[oi] = tt / dt;
+        s_i = s_p;
+    }
+    // the last row keeps the eliminated system values (ud 0, reduced diag, ld), as the reference leaves them
+}
+
+// ---------------------------------------------------- batched EOB splines
+// Three interpolants per binary over its trajectory knots t_k (units of M):
+// q = 0 phi, 1 Re(h22 e^{2 i phi}), 2 Im(h22 e^{2 i phi}).  coef [n][max_step][12]
+// holds (y, c1, c2, c3) per interpolant; scratch [n][3][max_step] holds slopes.
+__global__ __launch_bounds__(64) void k_spline_build(const double* __restrict__ dense, const double* __restrict__ knots,
+                                                     const int* __restrict__ num_steps, int n, int max_step, double* coef,
+                                                     double* scratch) {
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= 3 * n) return;
+    const int b = tid / 3, q = tid % 3;
+    const int len = num_steps[b];
+    const double* T = dense + (size_t)b * max_step * 5;   // t at stride 5
+    const double* Y = knots + (size_t)b * max_step * 3 + q;  // stride 3
+    double* C = coef + (size_t)b * max_step * 12 + 4 * q;    // stride 12
+    double* S = scratch + ((size_t)b * 3 + q) * max_step;
+    if (len < 3) {
+        for (int i = 0; i < len; ++i) { C[12 * i] = Y[3 * i]; C[12 * i + 1] = C[12 * i + 2] = C[12 * i + 3] = 0.0; }
+        return;
+    }
+    // forward elimination; keep modified diag in C[.+2], ud in C[.+1] temporarily
+    double pdg = 0.0, pud = 0.0, pb = 0.0;
+    for (int i = 0; i < len; ++i) {
+        int i0, i1, i2;
+        if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
+        else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
+        else { i0 = i - 1; i1 = i; i2 = i + 1; }
+        Row r = nak_row(i, len, T[5 * i0], T[5 * i1], T[5 * i2], Y[3 * i0], Y[3 * i1], Y[3 * i2]);
+        if (i > 0) {
+            const double w = r.ld / pdg;
+            r.dg = r.dg - w * pud;
+            r.b = r.b - w * pb;
+        }
+        C[12 * i + 1] = r.ud;
+        C[12 * i + 2] = r.dg;
+        S[i] = r.b;
+        pdg = r.dg; pud = r.ud; pb = r.b;
+    }
+    double sn = S[len - 1] / C[12 * (len - 1) + 2];
+    S[len - 1] = sn;
+    for (int i = len - 2; i >= 0; --i) {
+        sn = (S[i] - C[12 * i + 1] * sn) / C[12 * i + 2];
+        S[i] = sn;
+    }
+    for (int i = 0; i < len - 1; ++i) {
+        const double dt = T[5 * (i + 1)] - T[5 * i];
+        const double yi = Y[3 * i], yp = Y[3 * (i + 1)];
+        const double slope = (yp - yi) / dt;
+        const double si = S[i], sp = S[i + 1];
+        const double tt = (si + sp - 2 * slope) / dt;
+        C[12 * i] = yi;
+        C[12 * i + 1] = si;
+        C[12 * i + 2] = (slope - si) / dt - tt;
+        C[12 * i + 3] = tt / dt;
+    }
+    C[12 * (len - 1)] = Y[3 * (len - 1)];
+    C[12 * (len - 1) + 1] = C[12 * (len - 1) + 2] = C[12 * (len - 1) + 3] = 0.0;
+}
+
+// Uniform resample of h+ = amp * Re[(Re S + i Im S)(t) e^{-2 i phi(t)}] at
+// t_j = t_0 + j dT_M (dT_M = dt / (M MTSUN)), j < N_b = floor((t_last - t_0)/dT_M) + 1,
+// zero for N_b <= j < N.  Interval: searchsorted(t, t_j, 'right') - 1 over the knots
+// (eob/seobnrv4phm.py:241-272).  grid (blocks over j, binaries).
+__global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restrict__ dense, const double* __restrict__ coef,
+                                                           const int* __restrict__ num_steps, const double* __restrict__ amp,
+                                                           const double* __restrict__ dT_M, int n, int max_step, int N,
+                                                           double* __restrict__ hplus, int* __restrict__ nsamp) {
+    const int b = blockIdx.y;
+    const int len = num_steps[b];
+    const double* T = dense + (size_t)b * max_step * 5;
+    const double* C = coef + (size_t)b * max_step * 12;
+    const double t0 = T[0], tl = T[5 * (len - 1)];
+    const double dT = dT_M[b];
+    long long nb = (long long)((tl - t0) / dT) + 1;
+    if (nb > N) nb = N;
+    if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = (int)nb;
+    const double a = amp[b];
+    double* out = hplus + (size_t)b * N;
+    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) {
+        double val = 0.0;
+        if (j < nb) {
+            const double tj = t0 + (double)j * dT;
+            int lo = 0, hi = len;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (T[5 * mid] <= tj) lo = mid + 1;
+                else hi = mid;
+            }
+            int ind = lo - 1;
+            if (ind < 0) ind = 0;
+            const double x = tj - T[5 * ind];
+            const double x2 = x * x, x3 = x2 * x;
+            const double* c = C + 12 * ind;
+            const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
+            const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
+            const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
+            double sp, cp;
+            sincos(2.0 * ph, &sp, &cp);
+            // Re[(sr + i si)(cos 2ph - i sin 2ph)]
+            val = a * (sr * cp + si * sp);
+        }
+        out[j] = val;
+    }
+}
+
+// log L_b = -1/2 * 4 df * sum_k |d_k - h_k|^2 / S(f_k), S(f) = (f/f0)^-4 + 1 + (f/f0)^2,
+// 1/S(0) = 0.  Stage 1: per (chunk, binary) partial sums with wave shuffles + LDS;
+// chunks of one binary are adjacent in blockIdx.x so the data spectrum is read
+// once from HBM and reused from L2 by every binary.
+constexpr int kLikeBlock = 256;
+constexpr int kLikePerThread = 8;
+__global__ __launch_bounds__(kLikeBlock) void k_fd_likelihood_partial(const double2* __restrict__ ht, const double2* __restrict__ dt,
+                                                                      int n, int F, double df, double f0, double* __restrict__ partial,
+                                                                      int nchunks) {
+    const int b = blockIdx.x % n;
+    const int chunk = blockIdx.x / n;
+    const int base = chunk * kLikeBlock * kLikePerThread;
+    const double2* h = ht + (size_t)b * F;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < kLikePerThread; ++u) {
+        const int k = base + u * kLikeBlock + threadIdx.x;
+        if (k < F) {
+            const double2 hv = h[k];
+            const double2 dv = dt[k];
+            const double re = dv.x - hv.x, im = dv.y - hv.y;
+            const double x = df * (double)k / f0;
+            const double x2 = x * x, x4 = x2 * x2;
+            const double w = (k == 0) ? 0.0 : x4 / (1.0 + x4 + x4 * x2);   // 1 / S(f)
+            acc += (re * re + im * im) * w;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double red[kLikeBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kLikeBlock / 64; ++w) s += red[w];
+        partial[(size_t)b * nchunks + chunk] = s;
+    }
+}
+
+__global__ __launch_bounds__(64) void k_fd_likelihood_final(const double* __restrict__ partial, int n, int nchunks, double df,
+                                                            double* __restrict__ logl) {
+    const int b = blockIdx.x;
+    if (b >= n) return;
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < nchunks; c += 64) acc += partial[(size_t)b * nchunks + c];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (threadIdx.x == 0) logl[b] = -0.5 * 4.0 * df * acc;
+}
+
+// Drop-in all_in_one_likelihood: per 1024-bin block partial sums of
+// |d - (F+ hp + Fx hc) e^{-2 pi i f dt}|^2 / psd.  One 256-thread block covers
+// 1024 bins (4 per thread); grid (bins / 1024, nbin, nchan).
+__global__ __launch_bounds__(kBlock) void k_all_in_one_likelihood(double2* temp_sum, const double2* __restrict__ hp,
+                                                                  const double2* __restrict__ hc, const double2* __restrict__ data,
+                                                                  const double* __restrict__ psd, const double* __restrict__ Fp,
+                                                                  const double* __restrict__ Fc, const double* __restrict__ ts,
+                                                                  double df, int nbin, int nchan, int F) {
+    const int blk = blockIdx.x, b = blockIdx.y, c = blockIdx.z;
+    const double fp = Fp[c * nbin + b], fc = Fc[c * nbin + b], tsh = ts[c * nbin + b];
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int k = blk * 1024 + u * kBlock + threadIdx.x;
+        if (k < F) {
+            const double2 a = hp[(size_t)b * F + k], e = hc[(size_t)b * F + k], d = data[(size_t)c * F + k];
+            const double hr = fp * a.x + fc * e.x, hi = fp * a.y + fc * e.y;
+            double s, co;
+            sincos(-2.0 * kPi * (df * k) * tsh, &s, &co);
+            const double rr = d.x - (hr * co - hi * s), ri = d.y - (hr * s + hi * co);
+            acc += (rr * rr + ri * ri) / psd[(size_t)c * F + k];
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ double red[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) s += red[w];
+        temp_sum[((size_t)blk * nbin + b) * nchan + c] = make_double2(s, 0.0);
+    }
+}
+
+// hipFFT plans, cached per (N, batch, stream)
+std::mutex g_plan_mu;
+std::map<std::pair<long long, long long>, hipfftHandle> g_plans;
+
+}  // namespace
+
+extern "C" {
+
+int eobhip_interpolate_TD(const double* t_arr, const double* propArrays, double* B, double* upper_diag, double* diag, double* lower_diag,
+                          const int* lengths, int numBinAll, int nsubs, void* stream) {
+    EOBHIP_REQUIRE(numBinAll >= 0 && nsubs >= 0, "bad sizes");
+    const int ni = numBinAll * nsubs;
+    if (ni == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_interpolate_TD, dim3(blocks_for(ni, kBlock)), dim3(kBlock), 0, as_stream(stream), t_arr, propArrays, B,
+                       upper_diag, diag, lower_diag, lengths, numBinAll, ni);
+    return check_launch("interpolate_TD");
+}
+
+int eobhip_all_in_one_likelihood(double* temp_sum, const double* hp_fd, const double* hc_fd, const double* data, const double* psd,
+                                 const double* Fplus, const double* Fcross, const double* time_shift, double df, int num_bin_all,
+                                 int nchannels, int data_length, void* stream) {
+    EOBHIP_REQUIRE(num_bin_all >= 0 && nchannels >= 0 && data_length >= 0, "bad sizes");
+    EOBHIP_REQUIRE(num_bin_all <= 65535 && nchannels <= 65535, "num_bin_all and nchannels must be <= 65535");
+    if (num_bin_all == 0 || nchannels == 0 || data_length == 0) return EOBHIP_OK;
+    const int nblk = (data_length + 1023) / 1024;
+    hipLaunchKernelGGL(k_all_in_one_likelihood, dim3(nblk, num_bin_all, nchannels), dim3(kBlock), 0, as_stream(stream),
+                       reinterpret_cast<double2*>(temp_sum), reinterpret_cast<const double2*>(hp_fd),
+                       reinterpret_cast<const double2*>(hc_fd), reinterpret_cast<const double2*>(data), psd, Fplus, Fcross, time_shift,
+                       df, num_bin_all, nchannels, data_length);
+    return check_launch("all_in_one_likelihood");
+}
+
+int eobhip_spline_build_batched(const double* dense, const double* knots, const int* num_steps, int n, int max_step, double* coef,
+                                double* scratch, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && max_step >= 1, "bad sizes");
+    if (n == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_spline_build, dim3(blocks_for(3LL * n, 64)), dim3(64), 0, as_stream(stream), dense, knots, num_steps, n, max_step,
+                       coef, scratch);
+    return check_launch("spline_build");
+}
+
+int eobhip_resample_hplus(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
+                          int max_step, int N, double* hplus, int* nsamp, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && N >= 1 && n <= 65535, "bad sizes");
+    if (n == 0) return EOBHIP_OK;
+    int bx = (int)((N + (long long)kBlock * 4 - 1) / ((long long)kBlock * 4));
+    if (bx > 4096) bx = 4096;
+    hipLaunchKernelGGL(k_resample_hplus, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, n, max_step, N,
+                       hplus, nsamp);
+    return check_launch("resample_hplus");
+}
+
+// Batched real-to-complex FFT of n rows of N samples into n rows of N/2+1 bins (no dt factor,
+// as the reference's rfft, eob/seobnrv4phm.py:742-744).
+int eobhip_rfft_batched(const double* in, double* out, int n, int N, void* stream) {
+    EOBHIP_REQUIRE(n >= 1 && N >= 2, "bad sizes");
+    hipfftHandle plan;
+    {
+        std::lock_guard<std::mutex> g(g_plan_mu);
+        auto key = std::make_pair((long long)N, (long long)n);
+        auto itp = g_plans.find(key);
+        if (itp == g_plans.end()) {
+            int dims[1] = {N};
+            hipfftResult r = hipfftPlanMany(&plan, 1, dims, nullptr, 1, N, nullptr, 1, N / 2 + 1, HIPFFT_D2Z, n);
+            if (r != HIPFFT_SUCCESS) {
+                set_error("hipfftPlanMany failed (%d)", (int)r);
+                return EOBHIP_EHIP;
+            }
+            g_plans[key] = plan;
+        } else {
+            plan = itp->second;
+        }
+        hipfftResult r = hipfftSetStream(plan, as_stream(stream));
+        if (r != HIPFFT_SUCCESS) {
+            set_error("hipfftSetStream failed (%d)", (int)r);
+            return EOBHIP_EHIP;
+        }
+        r = hipfftExecD2Z(plan, const_cast<double*>(in), reinterpret_cast<hipfftDoubleComplex*>(out));
+        if (r != HIPFFT_SUCCESS) {
+            set_error("hipfftExecD2Z failed (%d)", (int)r);
+            return EOBHIP_EHIP;
+        }
+    }
+    return EOBHIP_OK;
+}
+
+int eobhip_fd_likelihood(const double* htilde, const double* dtilde, double* partial, int n, int F, double df, double f0,
+                         double* logl, void* stream) {
+    EOBHIP_REQUIRE(n >= 1 && F >= 1, "bad sizes");
+    const int per = kLikeBlock * kLikePerThread;
+    const int nchunks = (F + per - 1) / per;
+    const long long grid = (long long)nchunks * n;
+    EOBHIP_REQUIRE(grid < (1LL << 31), "grid too large");
+    hipLaunchKernelGGL(k_fd_likelihood_partial, dim3((unsigned)grid), dim3(kLikeBlock), 0, as_stream(stream),
+                       reinterpret_cast<const double2*>(htilde), reinterpret_cast<const double2*>(dtilde), n, F, df, f0, partial, nchunks);
+    int rc = check_launch("fd_likelihood_partial");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fd_likelihood_final, dim3(n), dim3(64), 0, as_stream(stream), partial, n, nchunks, df, logl);
+    return check_launch("fd_likelihood_final");
+}
+
+size_t eobhip_fd_likelihood_partial_doubles(int n, int F) {
+    const int per = kLikeBlock * kLikePerThread;
+    return (size_t)n * (size_t)((F + per - 1) / per);
+}
+
+}  // extern "C"

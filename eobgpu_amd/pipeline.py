@@ -1,0 +1,202 @@
+"""Batched, device-resident SEOBNR pipeline on MI355X.
+
+One ``EOBBatch`` holds the per-binary tables (Hamiltonian coefficients and the
+35-mode flux table), the initial state and the dense trajectory output in HBM.
+Every stage is a HIP kernel from libeobhip.so:
+
+  setup      additionalArgs -> tables             (SEOBNRv4PHM.setup_class, eob/seobnrv4phm.py:1922-1964)
+  initial    r0 -> (pphi0, pr0, omega0)            (circular start; IC_diss false position, src/EOB.cu:2666-2834)
+  trajectory persistent or plain DOP853 batch     (run_trajectory, eob/seobnrv4phm.py:2072-2136)
+  hlm22      (2,2) mode on the trajectory knots   (get_hlms, eob/seobnrv4phm.py:2138-2221)
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from ._lib import call, lib
+from .utils import current_stream
+
+MTSUN_SI = 4.925491025543575903411922162094833998e-6
+NADD = 80
+
+
+def _dbl_factorial(n):
+    out = 1
+    for k in range(n, 1, -2):
+        out *= k
+    return out
+
+
+def newton_prefixes(m1, m2):
+    """Newtonian multipole prefixes for the 35 flux modes (l = 2..8, m = 1..l).
+
+    Damour, Iyer & Nagar 2008 Eqs. 5-7, following EOBComputeNewtonMultipolePrefixes
+    (eob/seobnrv4phm.py:1703-1776).  Returns complex [35][n].
+    """
+    m1 = np.atleast_1d(np.asarray(m1, dtype=np.float64))
+    m2 = np.atleast_1d(np.asarray(m2, dtype=np.float64))
+    M = m1 + m2
+    x1, x2 = m1 / M, m2 / M
+    eta = m1 * m2 / (M * M)
+    out = np.zeros((35, m1.size), dtype=np.complex128)
+    q = 0
+    for l in range(2, 9):
+        for m in range(1, l + 1):
+            eps = (l + m) % 2
+            sign = 1 if m % 2 == 0 else -1
+            t1 = (m1 != m2) | (sign == 1)
+            cdeg = np.where(t1, x2 ** (l + eps - 1) + sign * x1 ** (l + eps - 1),
+                            -1.0 if l in (2, 3) else (-0.5 if l in (4, 5) else 0.0))
+            df = _dbl_factorial(2 * l + 1)
+            n = (1j * m) ** l
+            if eps == 0:
+                n = n * (8.0 * math.pi / df) * math.sqrt((l + 1) * (l + 2) / (l * (l - 1)))
+            else:
+                m2v = (2.0 * l + 1) * (l + 2) * (l * l - m * m) / ((2.0 * l - 1) * (l + 1) * l * (l - 1))
+                n = -n * (1j * 16.0 * math.pi / df) * math.sqrt(m2v)
+            out[q] = n * eta * cdeg
+            q += 1
+    return out
+
+
+def make_additional_args(m1, m2, chi1, chi2, omega0=None, K=0.0, d5=0.0, dSO=0.0, dSS=0.0, h22_calib=0.0):
+    """additionalArgs [80][n] as SEOBNRv4PHM.setup_class lays it out (masses scaled to M = 1)."""
+    m1 = np.atleast_1d(np.asarray(m1, dtype=np.float64))
+    m2 = np.atleast_1d(np.asarray(m2, dtype=np.float64))
+    M = m1 + m2
+    m1s, m2s = m1 / M, m2 / M
+    add = np.zeros((NADD, m1.size))
+    add[0], add[1], add[2], add[3] = m1s, m2s, chi1, chi2
+    add[4] = 0.0 if omega0 is None else omega0
+    add[5], add[6], add[7], add[8], add[9] = K, d5, dSO, dSS, h22_calib
+    pf = newton_prefixes(m1s, m2s)
+    add[10::2] = pf.real
+    add[11::2] = pf.imag
+    return add
+
+
+class EOBBatch:
+    """Device-resident tables and trajectory buffers for n binaries."""
+
+    def __init__(self, additional_args, max_step=4096, device="cuda", use_hm=1):
+        add = torch.as_tensor(np.ascontiguousarray(additional_args, dtype=np.float64), device=device)
+        self.n = int(add.shape[1])
+        self.device = device
+        self.max_step = int(max_step)
+        self.add = add
+        L = lib()
+        self.bindata = torch.empty(int(L.eobhip_bindata_bytes()) * self.n, dtype=torch.uint8, device=device)
+        self.modes = torch.empty(int(L.eobhip_modes_bytes()) * self.n, dtype=torch.uint8, device=device)
+        call("eobhip_bin_setup", add.data_ptr(), self.n, int(use_hm), self.bindata.data_ptr(), self.modes.data_ptr(), current_stream())
+        self.dense = torch.zeros((self.n, self.max_step, 5), dtype=torch.float64, device=device)
+        self.num_steps = torch.zeros(self.n, dtype=torch.int32, device=device)
+        self.stop_info = torch.zeros(self.n, dtype=torch.int32, device=device)
+        self.pass1_steps = torch.zeros(self.n, dtype=torch.int32, device=device)
+        self.attempts = torch.zeros(self.n, dtype=torch.int64, device=device)
+        self.scratch = torch.empty(int(L.eobhip_integrate_scratch_bytes(self.n)), dtype=torch.uint8, device=device)
+        self.cond = None
+
+    # ------------------------------------------------------------------ setup
+    def initial_conditions(self, r0, max_iter=1000, err=1e-12):
+        r0 = torch.as_tensor(np.broadcast_to(np.asarray(r0, dtype=np.float64), (self.n,)).copy(), device=self.device)
+        pphi0 = torch.empty(self.n, dtype=torch.float64, device=self.device)
+        pr0 = torch.empty_like(pphi0)
+        om0 = torch.empty_like(pphi0)
+        call("eobhip_initial_conditions", r0.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, pphi0.data_ptr(),
+             pr0.data_ptr(), om0.data_ptr(), int(max_iter), float(err), current_stream())
+        self.cond = torch.stack([r0, torch.zeros_like(r0), pr0, pphi0]).contiguous()
+        self.add[4] = om0
+        return pphi0, pr0, om0
+
+    def set_initial_state(self, cond):
+        self.cond = torch.as_tensor(np.ascontiguousarray(cond, dtype=np.float64), device=self.device).reshape(4, self.n).contiguous()
+
+    # ------------------------------------------------------------- trajectory
+    def integrate(self, rtol=1e-12, atol=1e-12, hInit=0.01, step_back=10.0, fine_step=0.05, two_pass=True, fix_step=False,
+                  mode="persistent", group=64, block=256, grid_groups=0):
+        """Run the batch.  Returns the driver iteration count (plain mode) or 0."""
+        if self.cond is None:
+            raise RuntimeError("initial state not set")
+        it = ctypes.c_longlong(0)
+        call("eobhip_integrate", self.cond.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, self.max_step,
+             float(rtol), float(atol), float(hInit), float(step_back), float(fine_step), int(bool(two_pass)), int(bool(fix_step)),
+             0 if mode == "persistent" else 1, int(group), int(block), int(grid_groups), self.dense.data_ptr(),
+             self.num_steps.data_ptr(), self.stop_info.data_ptr(), self.pass1_steps.data_ptr(), self.attempts.data_ptr(),
+             self.scratch.data_ptr(), ctypes.byref(it), current_stream())
+        return int(it.value)
+
+    def trajectory_host(self):
+        """(t [n][L], traj [n][4][L], num_steps) on the host, t in units of M, zero past num_steps."""
+        ns = self.num_steps.cpu().numpy()
+        Lmax = int(ns.max()) if self.n else 0
+        d = self.dense[:, :Lmax].cpu().numpy().copy()
+        for b in range(self.n):
+            d[b, ns[b]:] = 0.0
+        return d[:, :, 0], d[:, :, 1:].transpose(0, 2, 1), ns
+
+    # ----------------------------------------------------------- (2,2) knots
+    def hlm22_knots(self, out=None):
+        if out is None:
+            out = torch.zeros((self.n, self.max_step, 3), dtype=torch.float64, device=self.device)
+        call("eobhip_hlm22_knots", self.dense.data_ptr(), self.num_steps.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(),
+             self.n, self.max_step, out.data_ptr(), current_stream())
+        self.knots = out
+        return out
+
+    # ------------------------------------------------- splines and resample
+    def build_splines(self):
+        """Not-a-knot splines of phi, Re and Im of h22 e^{2 i phi} over each binary's knots."""
+        if getattr(self, "coef", None) is None:
+            self.coef = torch.empty((self.n, self.max_step, 12), dtype=torch.float64, device=self.device)
+            self.spline_scratch = torch.empty((self.n, 3, self.max_step), dtype=torch.float64, device=self.device)
+        call("eobhip_spline_build_batched", self.dense.data_ptr(), self.knots.data_ptr(), self.num_steps.data_ptr(), self.n,
+             self.max_step, self.coef.data_ptr(), self.spline_scratch.data_ptr(), current_stream())
+        return self.coef
+
+    def resample_hplus(self, total_mass_msun, dt, N, amp=None, out=None):
+        """h+ = amp Re(h22) on t_j = t_0 + j dt (seconds), j < N (zero past the last knot)."""
+        M = torch.as_tensor(np.broadcast_to(np.asarray(total_mass_msun, dtype=np.float64), (self.n,)).copy(), device=self.device)
+        dT_M = (dt / (M * MTSUN_SI)).contiguous()
+        if amp is None:
+            amp = torch.ones(self.n, dtype=torch.float64, device=self.device)
+        else:
+            amp = torch.as_tensor(np.broadcast_to(np.asarray(amp, dtype=np.float64), (self.n,)).copy(), device=self.device)
+        if out is None:
+            out = torch.empty((self.n, N), dtype=torch.float64, device=self.device)
+        self.nsamp = torch.zeros(self.n, dtype=torch.int32, device=self.device)
+        call("eobhip_resample_hplus", self.dense.data_ptr(), self.coef.data_ptr(), self.num_steps.data_ptr(), amp.data_ptr(),
+             dT_M.data_ptr(), self.n, self.max_step, int(N), out.data_ptr(), self.nsamp.data_ptr(), current_stream())
+        self._keep = (dT_M, amp)
+        return out
+
+
+def rfft_batched(h, out=None):
+    """Row-wise real-to-complex FFT (hipFFT D2Z), no dt factor (as the reference's rfft)."""
+    n, N = h.shape
+    if out is None:
+        out = torch.empty((n, N // 2 + 1), dtype=torch.complex128, device=h.device)
+    call("eobhip_rfft_batched", h.data_ptr(), out.data_ptr(), int(n), int(N), current_stream())
+    return out
+
+
+def fd_likelihood(htilde, dtilde, df, f0=200.0, partial=None, out=None):
+    """log L_b = -1/2 4 df sum_k |d_k - h_k|^2 / S(f_k), analytic S(f) = (f/f0)^-4 + 1 + (f/f0)^2."""
+    n, F = htilde.shape
+    if partial is None:
+        partial = torch.empty(int(lib().eobhip_fd_likelihood_partial_doubles(n, F)), dtype=torch.float64, device=htilde.device)
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=htilde.device)
+    call("eobhip_fd_likelihood", htilde.data_ptr(), dtilde.data_ptr(), partial.data_ptr(), int(n), int(F), float(df), float(f0),
+         out.data_ptr(), current_stream())
+    return out
+
+
+def analytic_psd(f, f0=200.0):
+    """S(f) = (f/f0)^-4 + 1 + (f/f0)^2 (S(0) = inf)."""
+    x = np.asarray(f, dtype=np.float64) / f0
+    with np.errstate(divide="ignore"):
+        return x ** -4.0 + 1.0 + x * x

@@ -1,0 +1,154 @@
+/*
+ * eobhip — MI355X (gfx950) HIP implementation of EOBGPU's batched SEOBNR hot
+ * path behind a C ABI.
+ *
+ * Conventions
+ *   - Every pointer argument is a device pointer unless its comment says "host".
+ *   - Layouts are the reference's: state arrays [4][numSys] flattened j*numSys+i,
+ *     additionalArgs [80][numSys] (rows 0..9 m1, m2, chi1, chi2, omega0, K, d5,
+ *     dSO, dSS, h22_calib; rows 10..79 Re/Im of the 35 Newtonian prefixes),
+ *     complex arrays are interleaved (re, im) doubles.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).  Calls
+ *     are asynchronous on that stream unless stated otherwise.
+ *   - Every entry point returns 0 (EOBHIP_OK) or an error code; the message of the
+ *     last failure on the calling thread is available from eobhip_last_error().
+ *     Nothing calls exit().
+ */
+#ifndef EOBHIP_H
+#define EOBHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EOBHIP_OK 0
+#define EOBHIP_EINVAL 1
+#define EOBHIP_EHIP 2
+#define EOBHIP_ENOMEM 3
+#define EOBHIP_EUNSUPPORTED 4
+
+const char* eobhip_last_error(void);
+int eobhip_version(void);
+
+/* The DOP853 tableau compiled into the library (host out, 12+144+12+12+3 doubles:
+ * c, a (row-major 12x12), b, e5, bhh). No reference counterpart; lets tests pin
+ * the constants against a published table. */
+int eobhip_dop853_tableau(double* out_host);
+
+/* ---------------------------------------------------------------- eob.pyx */
+/* evaluate_Ham_align_AD_wrap, include/EOB.hh:11; src/eob.pyx:92-102 */
+int eobhip_evaluate_Ham_align_AD(double* out, const double* r, const double* phi, const double* pr, const double* pphi,
+                                 const double* m_1, const double* m_2, const double* chi_1, const double* chi_2,
+                                 double K, double d5, double dSO, double dSS, int numSys, void* stream);
+/* grad_Ham_align_AD_wrap, include/EOB.hh:8; src/eob.pyx:72-79.  grad_out [4][numSys] */
+int eobhip_grad_Ham_align_AD(const double* arg, double* grad_out, const double* additionalArgs, int numSys, void* stream);
+/* hessian_Ham_align_AD_wrap, include/EOB.hh:9; src/eob.pyx:81-89.  hess_out [16][numSys], (a*4+b) */
+int eobhip_hessian_Ham_align_AD(const double* arg, double* hess_out, const double* additionalArgs, int numSys, void* stream);
+/* IC_cons_wrap, include/EOB.hh:13; src/eob.pyx:94-101.  res [2][numSys] = (dH/dpphi - omega0, dH/dr);
+ * args [4][numSys] is overwritten with (r, 0, 0, pphi) as the reference does */
+int eobhip_IC_cons(double* res, const double* x, double* args, const double* additionalArgs, double* grad_out, int numSys,
+                   void* stream);
+
+/* ------------------------------------------- eob.pyx SEOBNRv5Class (opaque handle) */
+typedef struct eobhip_seobnr* eobhip_seobnr_t;
+/* SEOBNRv5::SEOBNRv5 / ~SEOBNRv5, include/EOB.hh:372-374 */
+int eobhip_seobnr_create(eobhip_seobnr_t* out_handle);
+int eobhip_seobnr_destroy(eobhip_seobnr_t h);
+/* SEOBNRv5::update_information, include/EOB.hh:377; src/EOB.cu:3355-3374.
+ * All per-binary inputs [numSys]; newtonian_prefixes_hlms is kept by pointer
+ * (complex [5][numSys] for modes (2,2),(2,1),(3,3),(4,4),(5,5)) like the reference.
+ * Returns EOBHIP_EINVAL if already allocated (the reference prints and returns). */
+int eobhip_seobnr_update_information(eobhip_seobnr_t h, const double* m1, const double* m2, const double* eta,
+                                     const double* tplspin, const double* chi_S, const double* chi_A, int use_hm, int numSys,
+                                     const double* newtonian_prefixes_hlms, void* stream);
+/* SEOBNRv5::deallocate_information, include/EOB.hh:376 */
+int eobhip_seobnr_deallocate_information(eobhip_seobnr_t h);
+/* SEOBNRv5::RR_force_wrap, include/EOB.hh:378.  force_out [2][numSys]; omega taken from additionalArgs row 4 */
+int eobhip_seobnr_RR_force(eobhip_seobnr_t h, double* force_out, double* grad_out, const double* args,
+                           const double* additionalArgs, int numSys, const int* hCoeffs_index, void* stream);
+/* SEOBNRv5::ODE_Ham_align_AD_wrap, include/EOB.hh:380; src/EOB.cu:3486-3496.  k [4][numSys] */
+int eobhip_seobnr_ODE_Ham_align_AD(eobhip_seobnr_t h, const double* x, const double* arg, double* k,
+                                   const double* additionalArgs, int numSys, const int* hCoeffs_index, void* stream);
+/* SEOBNRv5::EOBGetSpinFactorizedWaveform_wrap, include/EOB.hh:384-401, for (l, m) = (2, 2);
+ * other modes return EOBHIP_EUNSUPPORTED.  Knot arrays [numSys][traj_length]; out complex. */
+int eobhip_seobnr_EOBGetSpinFactorizedWaveform(eobhip_seobnr_t h, double* out, const double* r, const double* phi,
+                                               const double* pr, const double* pphi, const double* v, const double* Hreal,
+                                               const double* eta, const double* vPhiInput, int l, int m,
+                                               const double* newtonian_prefix, double h22_calib, int numSys, int traj_length,
+                                               const int* hCoeffs_index, void* stream);
+
+/* ---------------------------------------------------------------- dopr853.pyx */
+/* dormandPrinceSteps_wrap, include/DOPR853.hh:6-27; src/DOPR853.cu:357-517.
+ * The 12 stages; the stage-11 result goes to k2, stage 12 to k3. */
+int eobhip_dormandPrinceSteps(double* x, double* xCurrent, double* solOld, double* h, double* arg, double* additionalArgs,
+                              double* k1, double* k2, double* k3, double* k4, double* k5, double* k6, double* k7,
+                              double* k8, double* k9, double* k10, double* ak_term_buffer, int nargs, int numEq,
+                              int num_add_args, eobhip_seobnr_t ode, const int* hCoeffs_index, void* stream);
+/* error_wrap, include/DOPR853.hh:29-43; src/DOPR853.cu:539-618 (rtol = atol = 1e-12 as in the reference) */
+int eobhip_error(double* err, const double* solOld, double* solNew, const double* h, const double* k1, const double* k2,
+                 const double* k3, const double* k6, const double* k7, const double* k8, const double* k9,
+                 const double* k10, int numEq, int nargs, void* stream);
+/* same with explicit tolerances (no reference counterpart: the reference hard-codes them, src/DOPR853.cu:7-8) */
+int eobhip_error_tol(double* err, const double* solOld, double* solNew, const double* h, const double* k1, const double* k2,
+                     const double* k3, const double* k6, const double* k7, const double* k8, const double* k9,
+                     const double* k10, int numEq, int nargs, double rtol, double atol, void* stream);
+/* controllerSuccess_wrap, include/DOPR853.hh:45-54; src/DOPR853.cu:675-762 (1-byte bool flags) */
+int eobhip_controllerSuccess(unsigned char* flagSuccess, const double* err, double* errOld, unsigned char* previousReject,
+                             double* h, const double* x, int numEq, int nargs, void* stream);
+
+/* ---------------------------------------------------------------- tdinterp.pyx */
+/* interpolate_TD (interpolate_TD_wrap), include/interpolate.hh:13-15; src/interpolate.cu:269-308.
+ * t_arr [maxlen][numBinAll], propArrays/B/diag/... [maxlen][nsubs*numBinAll]; outputs alias:
+ * upper_diag <- c1, diag <- c2, lower_diag <- c3 */
+int eobhip_interpolate_TD(const double* t_arr, const double* propArrays, double* B, double* upper_diag, double* diag,
+                          double* lower_diag, const int* lengths, int numBinAll, int nsubs, void* stream);
+/* all_in_one_likelihood_wrap, include/interpolate.hh:17; src/interpolate.cu:616-628.
+ * temp_sum complex [ceil(data_length/1024)][num_bin_all][nchannels] partial sums */
+int eobhip_all_in_one_likelihood(double* temp_sum, const double* hp_fd, const double* hc_fd, const double* data,
+                                 const double* psd, const double* Fplus, const double* Fcross, const double* time_shift,
+                                 double df, int num_bin_all, int nchannels, int data_length, void* stream);
+
+/* ------------------------------------------------ batched pipeline (MI355X design)
+ * No single reference counterpart: these compose SEOBNRv4PHM.setup_class
+ * (eob/seobnrv4phm.py:1922-1964), run_trajectory (:2072-2136) with
+ * DOPR853.integrate (eob/new_Dormand_horiz.py:424-778), get_hlms (:2138-2221)
+ * and the spline resample + likelihood (:2319-2372, :772-823) into device-resident
+ * batch kernels. */
+
+/* bytes per binary of the two device tables produced by eobhip_bin_setup */
+size_t eobhip_bindata_bytes(void);
+size_t eobhip_modes_bytes(void);
+/* per-binary Hamiltonian + flux tables from additionalArgs [80][n] */
+int eobhip_bin_setup(const double* additionalArgs, int n, int use_hm, void* bindata, void* modes, void* stream);
+
+/* Circular-orbit start at r0: pphi0 from dH/dr = 0, omega0 = dH/dpphi, and pr0 by the
+ * reference's IC_diss false position on [-3e-2, 0] (src/EOB.cu:2666-2834). */
+int eobhip_initial_conditions(const double* r0, const void* bindata, const void* modes, int n, double* pphi0, double* pr0,
+                              double* omega0, int max_iter, double err, void* stream);
+
+/* Integrate n binaries.  mode 0: one persistent work-stealing kernel (groups of `group`
+ * lanes pull binaries from an atomic queue); mode 1: one kernel launch per attempted
+ * step with on-device compaction of the active list (the reference's batching model).
+ * two_pass = 1 gives run_trajectory semantics (adaptive pass, walk back step_back,
+ * fixed-step pass of fine_step); two_pass = 0 one integrate() pass (fix_step as given).
+ * dense [n][max_step][5] (t, r, phi, pr, pphi); num_steps[n] = stored points;
+ * scratch: eobhip_integrate_scratch_bytes(n) bytes of device memory.
+ * iterations_out (host, may be NULL): driver iterations (plain mode) or 0. */
+size_t eobhip_integrate_scratch_bytes(int n);
+int eobhip_integrate(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol,
+                     double atol, double hInit, double step_back, double fine_step, int two_pass, int fix_step, int mode,
+                     int group, int block, int grid_groups, double* dense, int* num_steps, int* stop_info,
+                     int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out,
+                     void* stream);
+
+/* (2,2) mode on the trajectory knots of every binary (get_hlms for (2,2)):
+ * knots [n][max_step][3] = (phi, Re, Im) of h22 * exp(2 i phi); rows >= num_steps untouched */
+int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n,
+                       int max_step, double* knots, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EOBHIP_H */

@@ -1,0 +1,299 @@
+"""ORACLE — test infrastructure only.
+
+numpy/ctypes front end for the CPU restatement in ``oracle_eob.cpp`` (built into
+``oracle/build/liboracle.so`` by ``oracle/Makefile``) and for the reference's own
+``src/DOPR853.cu`` compiled unmodified into ``oracle/_ref/libdopr853_ref.so``.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
+import this module, and only as the checker / CPU baseline.  The shipped
+``eobgpu_amd`` package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libdopr853_ref.so")
+
+_lib = None
+_ref = None
+
+P = ctypes.POINTER(ctypes.c_double)
+PI32 = ctypes.POINTER(ctypes.c_int)
+PU8 = ctypes.POINTER(ctypes.c_ubyte)
+
+NADD = 80
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the oracle (and oracle/_ref when the reference tree exists)."""
+    out = subprocess.run(["make", "-C", HERE], capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + out.stdout + out.stderr)
+    if not quiet:
+        print(out.stdout)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def ref_lib():
+    """The reference's DOPR853.cu host build, or None if it was not built here."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        # lazy binding: the library's one unresolved symbol belongs to
+        # dormandPrinceSteps_wrap, which is never called from here
+        _ref = ctypes.CDLL(REF_PATH, mode=os.RTLD_LAZY)
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(P)
+
+
+def _pi(a):
+    return a.ctypes.data_as(PI32)
+
+
+def _c(a, dtype=np.float64):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def coeff_doubles() -> int:
+    return lib().orc_coeff_doubles()
+
+
+# --------------------------------------------------------------- physics API
+def evaluate_H(r, phi, pr, pphi, m1, m2, chi1, chi2, K=0.0, d5=0.0, dSO=0.0, dSS=0.0):
+    r, phi, pr, pphi, m1, m2, chi1, chi2 = [_c(x) for x in np.broadcast_arrays(r, phi, pr, pphi, m1, m2, chi1, chi2)]
+    n = r.size
+    out = np.zeros(n)
+    lib().orc_evaluate_H(_p(out), _p(r), _p(phi), _p(pr), _p(pphi), _p(m1), _p(m2), _p(chi1), _p(chi2),
+                         ctypes.c_double(K), ctypes.c_double(d5), ctypes.c_double(dSO), ctypes.c_double(dSS), n)
+    return out
+
+
+def grad_H(arg, add):
+    arg = _c(arg); add = _c(add)
+    n = arg.shape[-1]
+    g = np.zeros((4, n))
+    lib().orc_grad_H(_p(arg), _p(g), _p(add), n)
+    return g
+
+
+def hessian_H(arg, add):
+    arg = _c(arg); add = _c(add)
+    n = arg.shape[-1]
+    h = np.zeros((16, n))
+    lib().orc_hessian_H(_p(arg), _p(h), _p(add), n)
+    return h.reshape(4, 4, n)
+
+
+def newton_prefixes(m1, m2):
+    m1 = _c(np.atleast_1d(m1)); m2 = _c(np.atleast_1d(m2))
+    n = m1.size
+    out = np.zeros((35, n, 2))
+    lib().orc_newton_prefixes(_p(out), _p(m1), _p(m2), n)
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def make_additional_args(m1, m2, chi1, chi2, omega0=None, K=0.0, d5=0.0, dSO=0.0, dSS=0.0, h22_calib=0.0):
+    """additionalArgs [80][n] as SEOBNRv4PHM.setup_class builds it (eob/seobnrv4phm.py:1922-1964)."""
+    m1 = np.atleast_1d(np.asarray(m1, dtype=float)); m2 = np.atleast_1d(np.asarray(m2, dtype=float))
+    n = m1.size
+    M = m1 + m2
+    m1s, m2s = m1 / M, m2 / M
+    add = np.zeros((NADD, n))
+    add[0] = m1s; add[1] = m2s; add[2] = chi1; add[3] = chi2
+    add[4] = 0.0 if omega0 is None else omega0
+    add[5] = K; add[6] = d5; add[7] = dSO; add[8] = dSS; add[9] = h22_calib
+    pf = newton_prefixes(m1s, m2s)
+    add[10::2] = pf.real
+    add[11::2] = pf.imag
+    return add
+
+
+def fac_coeffs(add, use_hm=1):
+    add = _c(add)
+    n = add.shape[1]
+    m1, m2, c1, c2 = add[0], add[1], add[2], add[3]
+    eta = m1 * m2 / (m1 + m2) ** 2
+    chiS = 0.5 * (c1 + c2)
+    chiA = 0.5 * (c1 - c2)
+    a = (1 - 2 * eta) * chiS + (m1 - m2) / (m1 + m2) * chiA
+    out = np.zeros((n, coeff_doubles()))
+    lib().orc_fac_coeffs(_p(out), use_hm, _p(_c(m1)), _p(_c(m2)), _p(_c(eta)), _p(_c(a)), _p(_c(chiS)), _p(_c(chiA)), n)
+    return out
+
+
+def rr_force(arg, add, coeffs):
+    arg = _c(arg); add = _c(add); coeffs = _c(coeffs)
+    n = arg.shape[-1]
+    F = np.zeros((2, n))
+    idx = np.arange(n, dtype=np.int32)
+    lib().orc_rr_force(_p(F), _p(arg), _p(add), n, _p(coeffs), _pi(idx))
+    return F
+
+
+def rhs(arg, add, coeffs):
+    arg = _c(arg); add = _c(add); coeffs = _c(coeffs)
+    n = arg.shape[-1]
+    k = np.zeros((4, n))
+    idx = np.arange(n, dtype=np.int32)
+    lib().orc_rhs(_p(arg), _p(k), _p(add), n, _p(coeffs), _pi(idx))
+    return k
+
+
+def initial_conditions(r0, add, coeffs, max_iter=1000, err=1e-12):
+    """pphi0, pr0, omega0 for circular-orbit start at r0; add row 4 gets omega0."""
+    r0 = _c(np.atleast_1d(r0)); coeffs = _c(coeffs)
+    add = np.array(add, dtype=np.float64, order="C")
+    n = r0.size
+    pphi0 = np.zeros(n); pr0 = np.zeros(n); om0 = np.zeros(n)
+    lib().orc_initial_conditions(_p(r0), _p(add), _p(coeffs), n, _p(pphi0), _p(pr0), _p(om0), max_iter, ctypes.c_double(err))
+    return pphi0, pr0, om0, add
+
+
+# --------------------------------------------------------------- integrator
+def integrate(condBound, add, coeffs, hInit=0.01, max_step=400, tmax=1e13, fix_step=False, rtol=1e-12, atol=1e-12,
+              step_num=None, denseOutput=None, denseOutputLoc=None, batch_coupled=False):
+    """DOPR853.integrate semantics (eob/new_Dormand_horiz.py:424-778).
+
+    Returns (denseOutputLoc [max_step][n], denseOutput [max_step][4][n], step_num, stop_info, iterations).
+    """
+    condBound = _c(condBound); add = _c(add); coeffs = _c(coeffs)
+    n = condBound.shape[1]
+    restart = denseOutput is not None
+    if restart:
+        dO = np.array(denseOutput, dtype=np.float64, order="C")
+        dL = np.array(denseOutputLoc, dtype=np.float64, order="C")
+        sn = np.array(step_num, dtype=np.int32)
+        max_step = dO.shape[0]
+    else:
+        dO = np.zeros((max_step, 4, n)); dL = np.zeros((max_step, n)); sn = np.zeros(n, dtype=np.int32)
+    si = np.zeros(n, dtype=np.int32)
+    lib().orc_integrate.restype = ctypes.c_long
+    it = lib().orc_integrate(_p(condBound), _p(add), _p(coeffs), n, ctypes.c_double(hInit), max_step, ctypes.c_double(tmax),
+                             int(fix_step), ctypes.c_double(rtol), ctypes.c_double(atol), _pi(sn), _p(dO), _p(dL),
+                             int(restart), _pi(si), int(batch_coupled))
+    return dL, dO, sn, si, it
+
+
+def run_trajectory(condBound, add, coeffs, max_step=400, rtol=1e-12, atol=1e-12, hInit=0.01, step_back=10.0,
+                   fine_step=0.05, batch_coupled=False):
+    """SEOBNRv4PHM.run_trajectory two-pass semantics; t in units of M.
+
+    Returns (t [n][max_step], traj [n][4][max_step], num_steps, stop_info).
+    """
+    condBound = _c(condBound); add = _c(add); coeffs = _c(coeffs)
+    n = condBound.shape[1]
+    t = np.zeros((max_step, n)); traj = np.zeros((max_step, 4, n))
+    ns = np.zeros(n, dtype=np.int32); si = np.zeros(n, dtype=np.int32)
+    lib().orc_run_trajectory(_p(condBound), _p(add), _p(coeffs), n, max_step, ctypes.c_double(rtol), ctypes.c_double(atol),
+                             ctypes.c_double(hInit), ctypes.c_double(step_back), ctypes.c_double(fine_step),
+                             _p(t), _p(traj), _pi(ns), _pi(si), int(batch_coupled))
+    return t.T.copy(), traj.transpose(2, 1, 0).copy(), ns, si
+
+
+def error(solOld, h, ks, rtol=1e-12, atol=1e-12):
+    """Restated error norm; ks = dict of stage arrays k1,k2,k3,k6..k10 ([4][n])."""
+    solOld = _c(solOld); h = _c(h)
+    n = solOld.shape[1]
+    err = np.zeros(n); solNew = np.zeros_like(solOld)
+    lib().orc_error(_p(err), _p(solOld), _p(solNew), _p(h), *[_p(_c(ks[k])) for k in ("k1", "k2", "k3", "k6", "k7", "k8", "k9", "k10")],
+                    n, 4, ctypes.c_double(rtol), ctypes.c_double(atol))
+    return err, solNew
+
+
+def controller(err, errOld, prevReject, h):
+    err = _c(err); errOld = _c(errOld).copy(); h = _c(h).copy()
+    pr = np.array(prevReject, dtype=np.uint8)
+    flag = np.zeros(err.size, dtype=np.uint8)
+    lib().orc_controller(flag.ctypes.data_as(PU8), _p(err), _p(errOld), pr.ctypes.data_as(PU8), _p(h), err.size)
+    return flag.astype(bool), errOld, pr.astype(bool), h
+
+
+def ref_error(solOld, h, ks):
+    """The reference's error_wrap (src/DOPR853.cu:620-668), rtol = atol = 1e-12."""
+    r = ref_lib()
+    solOld = _c(solOld); h = _c(h)
+    n = solOld.shape[1]
+    err = np.zeros(n); solNew = np.zeros_like(solOld)
+    f = getattr(r, "_Z10error_wrapPdS_S_S_S_S_S_S_S_S_S_S_ii")
+    f(_p(err), _p(solOld), _p(solNew), _p(h), *[_p(_c(ks[k])) for k in ("k1", "k2", "k3", "k6", "k7", "k8", "k9", "k10")], n, 4)
+    return err, solNew
+
+
+def ref_controller(err, errOld, prevReject, h):
+    """The reference's controllerSuccess_wrap (src/DOPR853.cu:764-797)."""
+    r = ref_lib()
+    err = _c(err); errOld = _c(errOld).copy(); h = _c(h).copy()
+    pr = np.array(prevReject, dtype=np.uint8)
+    flag = np.zeros(err.size, dtype=np.uint8)
+    x = np.zeros(err.size)
+    f = getattr(r, "_Z22controllerSuccess_wrapPbPdS0_PbS0_S0_ii")
+    f(flag.ctypes.data_as(PU8), _p(err), _p(errOld), pr.ctypes.data_as(PU8), _p(h), _p(x), err.size, 4)
+    return flag.astype(bool), errOld, pr.astype(bool), h
+
+
+# ------------------------------------------------------------------ waveform
+def cgamma(z):
+    out = np.zeros(2)
+    lib().orc_cgamma(ctypes.c_double(z.real), ctypes.c_double(z.imag), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def ylm_equator(l, m, phi):
+    out = np.zeros(2)
+    lib().orc_ylm_equator(l, m, ctypes.c_double(phi), _p(out))
+    return out[0] + 1j * out[1]
+
+
+def get_hlm22(traj, lengths, add, coeffs, h22_calib=0.0):
+    """(2,2) mode on knots for traj [nbin][4][len] (eob/seobnrv4phm.py:2138-2221)."""
+    traj = _c(traj); lengths = _c(lengths, np.int32); add = _c(add); coeffs = _c(coeffs)
+    nbin, _, L = traj.shape
+    out = np.zeros((nbin, L, 2))
+    lib().orc_get_hlm22(_p(out), _p(traj), _pi(lengths), nbin, L, _p(add), _p(coeffs), ctypes.c_double(h22_calib))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+# ------------------------------------------------------------------- splines
+def interpolate_TD(t, y, lengths, nbin, nsubs):
+    """interpolate_TD: t [maxlen][nbin], y [maxlen][nsubs*nbin] -> (B, c1, c2, c3)."""
+    t = _c(t); y = _c(y); lengths = _c(lengths, np.int32)
+    B = np.zeros_like(y); ud = np.zeros_like(y); dg = np.zeros_like(y); ld = np.zeros_like(y)
+    lib().orc_interpolate_TD(_p(t), _p(y), _p(B), _p(ud), _p(dg), _p(ld), _pi(lengths), nbin, nsubs)
+    return B, ud, dg, ld
+
+
+def spline_eval(tnew, t, y, c1, c2, c3, length, deriv=0):
+    """One interpolant: knots t[:length], coefficient arrays 1-D."""
+    tnew = _c(tnew); t = _c(t); y = _c(y); c1 = _c(c1); c2 = _c(c2); c3 = _c(c3)
+    out = np.zeros(tnew.size)
+    lib().orc_spline_eval(_p(out), _p(tnew), tnew.size, _p(t), 1, _p(y), _p(c1), _p(c2), _p(c3), 1, int(length), int(deriv))
+    return out
+
+
+def likelihood_partials(hp, hc, data, psd, Fp, Fc, ts, df):
+    """all_in_one_likelihood partial sums [nblk][nbin][nchan] (complex)."""
+    hp = _c(np.asarray(hp, dtype=np.complex128).view(np.float64)); hc = _c(np.asarray(hc, dtype=np.complex128).view(np.float64))
+    data = _c(np.asarray(data, dtype=np.complex128).view(np.float64))
+    psd = _c(psd); Fp = _c(Fp); Fc = _c(Fc); ts = _c(ts)
+    nchan, F = psd.shape
+    nbin = Fp.shape[1]
+    nblk = (F + 1023) // 1024
+    out = np.zeros((nblk, nbin, nchan, 2))
+    lib().orc_likelihood(_p(out), _p(hp), _p(hc), _p(data), _p(psd), _p(Fp), _p(Fc), _p(ts), ctypes.c_double(df), nbin, nchan, F)
+    return out[..., 0] + 1j * out[..., 1]

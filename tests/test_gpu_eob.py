@@ -1,0 +1,145 @@
+"""GPU parity: Hamiltonian, gradient, Hessian, RHS, stepper and batched integrators
+against the CPU oracle on the same seeded inputs (reference semantics restated in
+oracle/oracle_eob.cpp)."""
+import numpy as np
+import pytest
+import torch
+
+from eobgpu_amd import configs
+from eobgpu_amd import pipeline as P
+from eobgpu_amd import pyEOB, pydopr853
+
+pytestmark = pytest.mark.gpu
+
+
+def _states(rng, n):
+    r = rng.uniform(3.0, 60.0, n)
+    pr = rng.uniform(-0.05, 0.0, n)
+    L = np.sqrt(r) * rng.uniform(0.9, 1.3, n)
+    return np.array([r, rng.uniform(0, 6.0, n), pr, L])
+
+
+def _add(rng, n):
+    q = rng.uniform(1.0, 10.0, n)
+    m1 = q / (1 + q)
+    m2 = 1 / (1 + q)
+    return P.make_additional_args(m1, m2, rng.uniform(-0.9, 0.9, n), rng.uniform(-0.9, 0.9, n))
+
+
+def test_hamiltonian_value_grad_hessian(oracle, dev):
+    rng = np.random.default_rng(11)
+    n = 257
+    arg = _states(rng, n)
+    add = _add(rng, n)
+    t_arg = torch.tensor(arg, device=dev).contiguous()
+    t_add = torch.tensor(add, device=dev).contiguous()
+    out = torch.zeros(n, dtype=torch.float64, device=dev)
+    cols = [t_arg[0], t_arg[1], t_arg[2], t_arg[3], t_add[0], t_add[1], t_add[2], t_add[3]]
+    cols = [c.contiguous() for c in cols]
+    pyEOB.evaluate_Ham_align_AD(out, *cols, 0.0, 0.0, 0.0, 0.0, n)
+    H_ref = oracle.evaluate_H(arg[0], arg[1], arg[2], arg[3], add[0], add[1], add[2], add[3])
+    np.testing.assert_allclose(out.cpu().numpy(), H_ref, rtol=1e-13, atol=0)
+
+    g = torch.zeros((4, n), dtype=torch.float64, device=dev)
+    pyEOB.grad_Ham_align_AD(t_arg, g, t_add, n)
+    g_ref = oracle.grad_H(arg, add)
+    np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=1e-11, atol=1e-15)
+
+    h = torch.zeros((16, n), dtype=torch.float64, device=dev)
+    pyEOB.hessian_Ham_align_AD(t_arg, h, t_add, n)
+    h_ref = oracle.hessian_H(arg, add).reshape(16, n)
+    np.testing.assert_allclose(h.cpu().numpy(), h_ref, rtol=1e-10, atol=1e-14)
+
+
+def test_ode_rhs_dropin(oracle, dev):
+    rng = np.random.default_rng(12)
+    n = 129
+    arg = _states(rng, n)
+    add = _add(rng, n)
+    co = oracle.fac_coeffs(add)
+    k_ref = oracle.rhs(arg, add, co)
+    m1, m2, c1, c2 = add[0], add[1], add[2], add[3]
+    eta = m1 * m2 / (m1 + m2) ** 2
+    chiS, chiA = 0.5 * (c1 + c2), 0.5 * (c1 - c2)
+    a = (1 - 2 * eta) * chiS + (m1 - m2) / (m1 + m2) * chiA
+    T = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev)
+    ode = pyEOB.SEOBNRv5Class()
+    pref = torch.zeros(2 * 5 * n, dtype=torch.float64, device=dev)
+    ode.update_information(T(m1), T(m2), T(eta), T(a), T(chiS), T(chiA), 1, n, pref)
+    k = torch.zeros((4, n), dtype=torch.float64, device=dev)
+    idx = torch.arange(n, dtype=torch.int32, device=dev)
+    ode.ODE_Ham_align_AD(torch.zeros(n, dtype=torch.float64, device=dev), T(arg), k, T(add), n, idx)
+    np.testing.assert_allclose(k.cpu().numpy(), k_ref, rtol=1e-11, atol=1e-18)
+    ode.deallocate_information()
+
+
+def test_error_controller_dropin(oracle, dev):
+    rng = np.random.default_rng(13)
+    n = 300
+    solOld = rng.normal(size=(4, n)) * 10
+    ks = {k: rng.normal(size=(4, n)) * 1e-6 for k in ("k1", "k2", "k3", "k6", "k7", "k8", "k9", "k10")}
+    h = rng.uniform(0.1, 50, n)
+    err_ref, sol_ref = oracle.error(solOld, h, ks)
+    T = lambda x: torch.tensor(np.ascontiguousarray(x), device=dev)
+    err = torch.zeros(n, dtype=torch.float64, device=dev)
+    sol = torch.zeros((4, n), dtype=torch.float64, device=dev)
+    pydopr853.error(err, T(solOld), sol, T(h), *[T(ks[k]) for k in ("k1", "k2", "k3", "k6", "k7", "k8", "k9", "k10")], n, 4)
+    np.testing.assert_allclose(sol.cpu().numpy(), sol_ref, rtol=1e-15, atol=1e-13)
+    np.testing.assert_allclose(err.cpu().numpy(), err_ref, rtol=1e-12)
+    errv = np.concatenate([err_ref[:100], np.array([0.0, 1.0, 1.0000001, 1e-9, 5e3]), rng.uniform(0.5, 2.0, n - 105)])
+    errOld = rng.uniform(1e-4, 1, n)
+    prev = rng.uniform(size=n) < 0.3
+    f_ref, eo_ref, pr_ref, h_ref = oracle.controller(errv, errOld, prev, h)
+    flag = torch.zeros(n, dtype=torch.uint8, device=dev)
+    eo = T(errOld)
+    prt = torch.tensor(prev.astype(np.uint8), device=dev)
+    ht = T(h)
+    pydopr853.controllerSuccess(flag, T(errv), eo, prt, ht, T(np.zeros(n)), n, 4)
+    assert np.array_equal(flag.cpu().numpy().astype(bool), f_ref)
+    assert np.array_equal(prt.cpu().numpy().astype(bool), pr_ref)
+    np.testing.assert_allclose(eo.cpu().numpy(), eo_ref, rtol=1e-15)
+    np.testing.assert_allclose(ht.cpu().numpy(), h_ref, rtol=1e-14)
+
+
+def _small_batch(oracle, n=16, seed=1):
+    p = configs.small_batch(n=n, seed=seed)
+    add = oracle.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
+    co = oracle.fac_coeffs(add)
+    pphi0, pr0, om0, add = oracle.initial_conditions(p["r0"], add, co)
+    cond = np.array([p["r0"], np.zeros(n), pr0, pphi0])
+    return add, co, cond
+
+
+def test_initial_conditions(oracle, dev):
+    p = configs.small_batch(n=16, seed=1)
+    add = P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
+    add_o = oracle.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
+    np.testing.assert_allclose(add, add_o, rtol=1e-14, atol=1e-16)
+    co = oracle.fac_coeffs(add_o)
+    pphi_r, pr_r, om_r, _ = oracle.initial_conditions(p["r0"], add_o, co)
+    b = P.EOBBatch(add, max_step=64, device=dev)
+    pphi, pr, om = b.initial_conditions(p["r0"])
+    np.testing.assert_allclose(pphi.cpu().numpy(), pphi_r, rtol=1e-13)
+    np.testing.assert_allclose(om.cpu().numpy(), om_r, rtol=1e-12)
+    np.testing.assert_allclose(pr.cpu().numpy(), pr_r, rtol=1e-8, atol=1e-13)
+
+
+@pytest.mark.parametrize("group", [64, 16, 1])
+@pytest.mark.parametrize("mode", ["persistent", "plain"])
+def test_trajectory_parity(oracle, dev, group, mode):
+    add, co, cond = _small_batch(oracle)
+    max_step = 4096
+    rtol = 1e-10
+    t_r, traj_r, ns_r, si_r = oracle.run_trajectory(cond, add, co, max_step=max_step, rtol=rtol, atol=rtol)
+    b = P.EOBBatch(add, max_step=max_step, device=dev)
+    b.set_initial_state(cond)
+    b.integrate(rtol=rtol, atol=rtol, mode=mode, group=group)
+    torch.cuda.synchronize()
+    t, traj, ns = b.trajectory_host()
+    si = b.stop_info.cpu().numpy()
+    assert np.array_equal(ns, ns_r), (ns, ns_r)
+    assert np.array_equal(si, si_r)
+    L = ns.max()
+    np.testing.assert_allclose(t, t_r[:, :L], rtol=1e-9, atol=1e-9)
+    for c in range(4):
+        np.testing.assert_allclose(traj[:, c], traj_r[:, c, :L], rtol=1e-9, atol=1e-12)
