@@ -40,7 +40,7 @@ struct IntegParams {
     int nbin, max_step;
     double rtol, atol, hInit, step_back, fine_step, tmax;
     int two_pass;             // 1: run_trajectory; 0: one DOPR853.integrate pass
-    int fix_step;             // single-pass mode only
+    int fix_step;             // first (or only) pass at fixed step hInit (the reference: off)
     unsigned long long max_attempts;  // per binary, both passes; ends runaway step-size loops
 };
 
@@ -356,7 +356,8 @@ __device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int 
             return;   // k1 is stale for the new state
         }
     }
-    const bool fixed = (s.pass == 2) || (!P.two_pass && P.fix_step);
+    // pass 2 is always fixed-step; fix_step makes the first (or only) pass fixed-step too
+    const bool fixed = (s.pass == 2) || P.fix_step;
     attempt_step<G>(P, s, bd, mc, k1, gl, fixed);
 }
 

@@ -170,6 +170,12 @@ __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restr
                                                            double* __restrict__ hplus, int* __restrict__ nsamp) {
     const int b = blockIdx.y;
     const int len = num_steps[b];
+    double* out = hplus + (size_t)b * N;
+    if (len < 2) {
+        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) out[j] = 0.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = 0;
+        return;
+    }
     const double* T = dense + (size_t)b * max_step * 5;
     const double* C = coef + (size_t)b * max_step * 12;
     const double t0 = T[0], tl = T[5 * (len - 1)];
@@ -178,7 +184,6 @@ __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restr
     if (nb > N) nb = N;
     if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = (int)nb;
     const double a = amp[b];
-    double* out = hplus + (size_t)b * N;
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) {
         double val = 0.0;
         if (j < nb) {

@@ -1,0 +1,41 @@
+"""Drop-in for the reference's ``eob.pytdinterp`` Cython module (src/tdinterp.pyx).
+
+Array arguments are GPU arrays; complex arrays are complex128.  Calls are
+synchronous like the reference's wraps.
+"""
+from __future__ import annotations
+
+from ._lib import call
+from .utils import current_stream, pointer_adjust, sync
+
+__all__ = ["interpolate_TD_wrap", "all_in_one_likelihood", "TDInterp_wrap2", "TDInterp_wrap"]
+
+
+@pointer_adjust
+def interpolate_TD_wrap(t_arr, propArrays, B, upper_diag, diag, lower_diag, lengths, numBinAll, nsubs):
+    """src/tdinterp.pyx:20-37 -> interpolate_TD (src/interpolate.cu:269-308).
+
+    Outputs alias as in the reference: upper_diag <- c1, diag <- c2, lower_diag <- c3.
+    """
+    call("eobhip_interpolate_TD", t_arr, propArrays, B, upper_diag, diag, lower_diag, lengths, int(numBinAll), int(nsubs),
+         current_stream())
+    sync()
+
+
+@pointer_adjust
+def all_in_one_likelihood(temp_sum, hp_fd, hc_fd, data, psd, Fplus, Fcross, time_shift, df, num_bin_all, nchannels, data_length):
+    """src/tdinterp.pyx:70-88 -> all_in_one_likelihood_wrap (src/interpolate.cu:616-628)."""
+    call("eobhip_all_in_one_likelihood", temp_sum, hp_fd, hc_fd, data, psd, Fplus, Fcross, time_shift, float(df), int(num_bin_all),
+         int(nchannels), int(data_length), current_stream())
+    sync()
+
+
+def TDInterp_wrap2(*args, **kwargs):
+    """src/tdinterp.pyx:55-68 (TD2 mode combine).  Dead on the reference's main path
+    (eob/seobnrv4phm.py evaluates splines with cupy); use EOBBatch.resample_hplus."""
+    raise NotImplementedError("TDInterp_wrap2 is not provided; see eobgpu_amd.pipeline.EOBBatch.resample_hplus")
+
+
+def TDInterp_wrap(*args, **kwargs):
+    """src/tdinterp.pyx:39-53 (pointer-array TD, unused by the reference)."""
+    raise NotImplementedError("TDInterp_wrap is dead code in the reference and is not provided")

@@ -148,6 +148,26 @@ int eobhip_integrate(const double* condBound, const void* bindata, const void* m
 int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n,
                        int max_step, double* knots, void* stream);
 
+/* Not-a-knot splines of the three knot series (phi and Re, Im of h22 e^{2 i phi}) per
+ * binary (CubicSplineInterpolantTD / interpolate_TD semantics, src/interpolate.cu:11-308).
+ * coef [n][max_step][12] = (y, c1, c2, c3) x 3; scratch [n][3][max_step] doubles. */
+int eobhip_spline_build_batched(const double* dense, const double* knots, const int* num_steps, int n, int max_step,
+                                double* coef, double* scratch, void* stream);
+/* h+ = amp * Re[h22(t_j)] on t_j = t_0 + j dT_M (dT_M [n] in units of M), j < N; zero past the
+ * last knot; nsamp[n] = samples covered by the knots (eob/seobnrv4phm.py:2319-2372). */
+int eobhip_resample_hplus(const double* dense, const double* coef, const int* num_steps, const double* amp,
+                          const double* dT_M, int n, int max_step, int N, double* hplus, int* nsamp, void* stream);
+/* Row-wise real-to-complex FFT, n rows of N doubles -> n rows of N/2+1 complex (hipFFT D2Z;
+ * like the reference's rfft without a dt factor, eob/seobnrv4phm.py:742-744). */
+int eobhip_rfft_batched(const double* in, double* out, int n, int N, void* stream);
+/* log L_b = -1/2 * 4 df * sum_k |d_k - h_b,k|^2 / S(f_k) with the analytic
+ * S(f) = (f/f0)^-4 + 1 + (f/f0)^2 (1/S(0) = 0): the reduction of all_in_one_likelihood
+ * plus the host finish of get_ll (eob/seobnrv4phm.py:772-823) in two device passes.
+ * partial: eobhip_fd_likelihood_partial_doubles(n, F) doubles. */
+int eobhip_fd_likelihood(const double* htilde, const double* dtilde, double* partial, int n, int F, double df, double f0,
+                         double* logl, void* stream);
+size_t eobhip_fd_likelihood_partial_doubles(int n, int F);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,12 @@ def coeff_doubles() -> int:
     return lib().orc_coeff_doubles()
 
 
+def tableau():
+    out = np.zeros(12 + 144 + 12 + 12 + 3)
+    lib().orc_tableau(_p(out))
+    return out
+
+
 # --------------------------------------------------------------- physics API
 def evaluate_H(r, phi, pr, pphi, m1, m2, chi1, chi2, K=0.0, d5=0.0, dSO=0.0, dSS=0.0):
     r, phi, pr, pphi, m1, m2, chi1, chi2 = [_c(x) for x in np.broadcast_arrays(r, phi, pr, pphi, m1, m2, chi1, chi2)]
@@ -191,7 +197,7 @@ def integrate(condBound, add, coeffs, hInit=0.01, max_step=400, tmax=1e13, fix_s
 
 
 def run_trajectory(condBound, add, coeffs, max_step=400, rtol=1e-12, atol=1e-12, hInit=0.01, step_back=10.0,
-                   fine_step=0.05, batch_coupled=False):
+                   fine_step=0.05, batch_coupled=False, fix_step1=False):
     """SEOBNRv4PHM.run_trajectory two-pass semantics; t in units of M.
 
     Returns (t [n][max_step], traj [n][4][max_step], num_steps, stop_info).
@@ -202,7 +208,7 @@ def run_trajectory(condBound, add, coeffs, max_step=400, rtol=1e-12, atol=1e-12,
     ns = np.zeros(n, dtype=np.int32); si = np.zeros(n, dtype=np.int32)
     lib().orc_run_trajectory(_p(condBound), _p(add), _p(coeffs), n, max_step, ctypes.c_double(rtol), ctypes.c_double(atol),
                              ctypes.c_double(hInit), ctypes.c_double(step_back), ctypes.c_double(fine_step),
-                             _p(t), _p(traj), _pi(ns), _pi(si), int(batch_coupled))
+                             _p(t), _p(traj), _pi(ns), _pi(si), int(batch_coupled), int(fix_step1))
     return t.T.copy(), traj.transpose(2, 1, 0).copy(), ns, si
 
 
@@ -242,7 +248,7 @@ def ref_controller(err, errOld, prevReject, h):
     pr = np.array(prevReject, dtype=np.uint8)
     flag = np.zeros(err.size, dtype=np.uint8)
     x = np.zeros(err.size)
-    f = getattr(r, "_Z22controllerSuccess_wrapPbPdS0_PbS0_S0_ii")
+    f = getattr(r, "_Z22controllerSuccess_wrapPbPdS0_S_S0_S0_ii")
     f(flag.ctypes.data_as(PU8), _p(err), _p(errOld), pr.ctypes.data_as(PU8), _p(h), _p(x), err.size, 4)
     return flag.astype(bool), errOld, pr.astype(bool), h
 

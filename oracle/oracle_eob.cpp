@@ -197,6 +197,17 @@ extern "C" {
 
 int orc_coeff_doubles() { return (int)(sizeof(FacCoeffs) / sizeof(double)); }
 
+// c[12], a[12][12], b[12], e5[12], bhh[3] (the product exports the same layout)
+void orc_tableau(double* out) {
+    int o = 0;
+    for (int i = 0; i < 12; ++i) out[o++] = kC[i];
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) out[o++] = kA[i][j];
+    for (int i = 0; i < 12; ++i) out[o++] = kB[i];
+    for (int i = 0; i < 12; ++i) out[o++] = kE5[i];
+    out[o++] = kBHH1; out[o++] = kBHH2; out[o++] = kBHH3;
+}
+
 // src/EOB.cu:109-137
 void orc_evaluate_H(double* out, const double* r, const double* phi, const double* pr, const double* pphi,
                     const double* m1, const double* m2, const double* chi1, const double* chi2,
@@ -460,14 +471,16 @@ long orc_integrate(const double* condBound, const double* add, const double* coe
 // SEOBNRv4PHM.run_trajectory (eob/seobnrv4phm.py:2072-2136): an adaptive pass,
 // then a walk back by step_back (in M) and a fixed-step pass of fine_step.
 // t_out: [max_step][n] (units of M), traj: [max_step][4][n]; num_steps[n] is
-// the number of stored points (accepted steps of pass 2 + 1).
+// the number of stored points (accepted steps of pass 2 + 1).  fix_step1 = 1
+// runs the first pass at fixed step hInit (the reference's pass 1 is adaptive).
 void orc_run_trajectory(const double* condBound, const double* add, const double* coeffs, int n,
                         int max_step, double rtol, double atol, double hInit, double step_back, double fine_step,
-                        double* t_out, double* traj, int* num_steps, int* stop_info, int batch_coupled) {
+                        double* t_out, double* traj, int* num_steps, int* stop_info, int batch_coupled, int fix_step1) {
     std::vector<int> sn(n, 0);
     std::memset(t_out, 0, sizeof(double) * (size_t)max_step * n);
     std::memset(traj, 0, sizeof(double) * (size_t)max_step * 4 * n);
-    orc_integrate(condBound, add, coeffs, n, hInit, max_step, 1e13, 0, rtol, atol, sn.data(), traj, t_out, 0, stop_info, batch_coupled);
+    orc_integrate(condBound, add, coeffs, n, hInit, max_step, 1e13, fix_step1, rtol, atol, sn.data(), traj, t_out, 0, stop_info,
+                  batch_coupled);
     for (int i = 0; i < n; ++i) {
         // t_stop is t[num_steps - 1]: the reference finds the last non-zero row of
         // t[1:] and then indexes the unshifted t with it (eob/seobnrv4phm.py:2095-2097)

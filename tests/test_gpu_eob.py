@@ -124,22 +124,85 @@ def test_initial_conditions(oracle, dev):
     np.testing.assert_allclose(pr.cpu().numpy(), pr_r, rtol=1e-8, atol=1e-13)
 
 
-@pytest.mark.parametrize("group", [64, 16, 1])
-@pytest.mark.parametrize("mode", ["persistent", "plain"])
-def test_trajectory_parity(oracle, dev, group, mode):
-    add, co, cond = _small_batch(oracle)
-    max_step = 4096
-    rtol = 1e-10
-    t_r, traj_r, ns_r, si_r = oracle.run_trajectory(cond, add, co, max_step=max_step, rtol=rtol, atol=rtol)
+def _run(add, cond, dev, max_step, **kw):
     b = P.EOBBatch(add, max_step=max_step, device=dev)
     b.set_initial_state(cond)
-    b.integrate(rtol=rtol, atol=rtol, mode=mode, group=group)
+    b.integrate(**kw)
     torch.cuda.synchronize()
     t, traj, ns = b.trajectory_host()
-    si = b.stop_info.cpu().numpy()
+    return t, traj, ns, b.stop_info.cpu().numpy(), b.pass1_steps.cpu().numpy()
+
+
+# Deterministic step sequence: the first pass at fixed h = 5 M, then the
+# reference's walk-back and fixed 0.05 M pass.  Accepted-step counts, stop codes
+# and the restart row must be identical; states agree at matched steps.
+@pytest.mark.parametrize("group", [64, 16, 1])
+@pytest.mark.parametrize("mode", ["persistent", "plain"])
+def test_trajectory_fixed_step_parity(oracle, dev, group, mode):
+    add, co, cond = _small_batch(oracle)
+    max_step = 8192
+    t_r, traj_r, ns_r, si_r = oracle.run_trajectory(cond, add, co, max_step=max_step, rtol=1e-10, atol=1e-10, hInit=5.0,
+                                                    fix_step1=True)
+    t, traj, ns, si, _ = _run(add, cond, dev, max_step, rtol=1e-10, atol=1e-10, hInit=5.0, fix_step=True, mode=mode, group=group)
     assert np.array_equal(ns, ns_r), (ns, ns_r)
-    assert np.array_equal(si, si_r)
+    assert np.array_equal(si, si_r), (si, si_r)
     L = ns.max()
-    np.testing.assert_allclose(t, t_r[:, :L], rtol=1e-9, atol=1e-9)
+    np.testing.assert_allclose(t, t_r[:, :L], rtol=1e-11, atol=1e-9)
     for c in range(4):
-        np.testing.assert_allclose(traj[:, c], traj_r[:, c, :L], rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(traj[:, c], traj_r[:, c, :L], rtol=1e-9, atol=1e-13)
+
+
+# The reference's adaptive controller (4th-power error norm with alpha = 1/8)
+# sits on its accept/reject boundary at almost every step, so the accepted-step
+# sequence is not reproducible between two floating-point implementations (the
+# oracle itself changes its step counts when one input moves by 1e-15).  The
+# adaptive trajectories are compared as solutions: at matched times, within
+# 10x the spread the oracle shows under that perturbation.
+def test_trajectory_adaptive_matched_times(oracle, dev):
+    from scipy.interpolate import CubicSpline
+    add, co, cond = _small_batch(oracle)
+    max_step = 4096
+    dL, dO, sn, si, _ = oracle.integrate(cond, add, co, max_step=max_step, rtol=1e-10, atol=1e-10)
+    t, traj, ns, sig, _ = _run(add, cond, dev, max_step, rtol=1e-10, atol=1e-10, two_pass=False, group=64)
+    steps_gpu = ns - 1
+    assert np.all(np.abs(steps_gpu - sn) <= 0.25 * sn), (steps_gpu, sn)
+    assert set(sig.tolist()) <= {3, 4, 5} and set(si.tolist()) <= {3, 4, 5}
+    tol = {0: 1e-4, 1: 5e-6, 3: 2e-5}
+    for b in range(cond.shape[1]):
+        ta, ya = dL[:sn[b] + 1, b], dO[:sn[b] + 1, :, b]
+        tb, yb = t[b, :ns[b]], traj[b, :, :ns[b]]
+        tt = np.linspace(0.0, 0.98 * min(ta[-1], tb[-1]), 1500)
+        for c, rt in tol.items():
+            fa = CubicSpline(ta, ya[:, c])(tt)
+            fb = CubicSpline(tb, yb[c])(tt)
+            rel = np.max(np.abs(fa - fb) / np.abs(fa).max())
+            assert rel < rt, (b, c, rel)
+
+
+@pytest.mark.parametrize("group", [64, 16, 1])
+def test_persistent_equals_plain(oracle, dev, group):
+    """Same device arithmetic in both drivers: identical adaptive trajectories."""
+    add, co, cond = _small_batch(oracle)
+    a = _run(add, cond, dev, 4096, rtol=1e-10, atol=1e-10, mode="persistent", group=group)
+    b = _run(add, cond, dev, 4096, rtol=1e-10, atol=1e-10, mode="plain", group=group)
+    assert np.array_equal(a[2], b[2]) and np.array_equal(a[3], b[3])
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_alone_and_in_the_mix(oracle, dev):
+    """A binary's result does not depend on the batch it runs in (order, size)."""
+    add, co, cond = _small_batch(oracle)
+    full = _run(add, cond, dev, 4096, rtol=1e-10, atol=1e-10, group=64)
+    perm = np.random.default_rng(3).permutation(cond.shape[1])
+    mixed = _run(add[:, perm], cond[:, perm], dev, 4096, rtol=1e-10, atol=1e-10, group=64, grid_groups=5)
+    for j, b in enumerate(perm):
+        assert full[2][b] == mixed[2][j]
+        L = full[2][b]
+        np.testing.assert_array_equal(full[0][b, :L], mixed[0][j, :L])
+        np.testing.assert_array_equal(full[1][b, :, :L], mixed[1][j, :, :L])
+    for b in (0, 7):
+        alone = _run(add[:, b:b + 1], cond[:, b:b + 1], dev, 4096, rtol=1e-10, atol=1e-10, group=64)
+        L = full[2][b]
+        assert alone[2][0] == L
+        np.testing.assert_array_equal(alone[1][0, :, :L], full[1][b, :, :L])

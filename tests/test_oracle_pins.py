@@ -1,0 +1,314 @@
+"""CPU tests that pin the oracle (oracle/oracle_eob.cpp) to things other than itself:
+
+  - the reference's own src/DOPR853.cu, compiled unmodified into oracle/_ref
+    (error norm and controller, bit for bit);
+  - the published Hairer-Wanner DOP853 tableau (as shipped in scipy.integrate);
+  - the Kerr test-mass limit of the Hamiltonian;
+  - published post-Newtonian EOB potentials: A(u) through 5.5PN
+    (Bini-Damour 2013; Damour-Jaranowski-Schaefer 2015: 4PN a5, the 5PN log and
+    the 5.5PN u^(13/2) terms), Dbar(u) through 4PN, the NLO spin-orbit
+    gyro-gravitomagnetic factors (Damour-Jaranowski-Schaefer 2008);
+  - finite differences of H for the AD gradient and Hessian;
+  - the PN energy flux (quadrupole normalization + 1PN + tail) for the factorized flux;
+  - scipy CubicSpline (not-a-knot) for the spline build, numpy for the likelihood,
+    scipy.special for Gamma(z) and Y_lm.
+Coefficients beyond those orders (e.g. the N3LO spin-orbit tables, the pr^6, pr^8
+Q terms, the rho_lm tables past the orders checked by the flux expansion) have
+no independent check here: parity unpinned.
+"""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.special as sps
+from scipy.interpolate import CubicSpline
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _nonspinning_add(oracle, q, chi1=0.0, chi2=0.0):
+    m1, m2 = q / (1 + q), 1 / (1 + q)
+    return oracle.make_additional_args(np.array([m1]), np.array([m2]), np.array([chi1]), np.array([chi2]))
+
+
+def _Heff(oracle, add, r, pr, L):
+    m1, m2 = add[0, 0], add[1, 0]
+    nu = m1 * m2 / (m1 + m2) ** 2
+    H = oracle.evaluate_H(np.atleast_1d(r), 0 * np.atleast_1d(r), np.atleast_1d(pr) + 0 * np.atleast_1d(r), np.atleast_1d(L) + 0 * np.atleast_1d(r),
+                          m1, m2, add[2, 0], add[3, 0])
+    return ((nu * H) ** 2 - 1) / (2 * nu) + 1, H, nu
+
+
+# --------------------------------------------------------------- DOP853
+def test_tableau_matches_scipy(oracle):
+    import scipy.integrate._ivp.dop853_coefficients as d
+    tab = oracle.tableau()
+    c, a, b, e5, bhh = tab[:12], tab[12:156].reshape(12, 12), tab[156:168], tab[168:180], tab[180:]
+    np.testing.assert_array_equal(c, d.C[:12])
+    np.testing.assert_array_equal(a, d.A[:12, :12])
+    np.testing.assert_array_equal(b, d.B[:12])
+    np.testing.assert_array_equal(e5, d.E5[:12])
+    # E3 = B - bhh on stages 1, 9, 12 (scipy subtracts the same published bhh values)
+    e3 = d.B[:12].copy()
+    e3[0] -= bhh[0]; e3[8] -= bhh[1]; e3[11] -= bhh[2]
+    np.testing.assert_array_equal(e3, d.E3[:12])
+
+
+def test_dop853_order_on_linear_ode():
+    """The tableau integrates y' = y to 8th order: one step of size h has error ~ h^9."""
+    import scipy.integrate._ivp.dop853_coefficients as d
+    errs = []
+    for h in (0.4, 0.2):
+        k = np.zeros(12)
+        for s in range(12):
+            k[s] = 1.0 + h * np.dot(d.A[s, :s], k[:s])
+        y1 = 1.0 + h * np.dot(d.B[:12], k)
+        errs.append(abs(y1 - math.exp(h)))
+    assert 7.5 < math.log2(errs[0] / errs[1]) < 10.5
+
+
+def test_error_norm_and_controller_match_reference_build(oracle):
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref was not built (reference tree absent when building)")
+    rng = np.random.default_rng(21)
+    n = 500
+    solOld = rng.normal(size=(4, n)) * np.array([[20.0], [300.0], [1e-3], [4.0]])
+    ks = {k: rng.normal(size=(4, n)) * 1e-4 for k in ("k1", "k2", "k3", "k6", "k7", "k8", "k9", "k10")}
+    h = rng.uniform(0.01, 300, n)
+    e_o, s_o = oracle.error(solOld, h, ks)
+    e_r, s_r = oracle.ref_error(solOld, h, ks)
+    np.testing.assert_array_equal(s_o, s_r)
+    np.testing.assert_allclose(e_o, e_r, rtol=4e-16, atol=0)
+    err = np.concatenate([e_r[:200], [0.0, 1.0, np.nextafter(1.0, 2.0), 1e-300, 1e300], rng.uniform(0.2, 3.0, n - 205)])
+    errOld = rng.uniform(1e-5, 2, n)
+    prev = rng.uniform(size=n) < 0.4
+    a = oracle.controller(err, errOld, prev, h)
+    b = oracle.ref_controller(err, errOld, prev, h)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+# ----------------------------------------------------------- Hamiltonian
+def test_kerr_test_mass_limit(oracle):
+    """nu -> 0: Heven is the equatorial Kerr geodesic Hamiltonian; Hodd is Kerr frame
+    dragging plus the reference's nu-independent SO calibration term (1701/512) a L / r^4."""
+    q = 1e7
+    for a in (0.0, 0.5, -0.8):
+        add = _nonspinning_add(oracle, q, chi1=a)
+        X1 = add[0, 0]
+        aa = a * X1
+        for r, pr, L in ((10.0, 0.01, 3.5), (6.0, -0.05, 3.2), (30.0, 0.0, 5.6)):
+            Heff, H, nu = _Heff(oracle, add, r, pr, L)
+            Delta = r * r - 2 * r + aa * aa
+            Lam = (r * r + aa * aa) ** 2 - aa * aa * Delta
+            alpha2 = Delta * r * r / Lam
+            hk = 2 * aa * r / Lam * L + math.sqrt(alpha2 * (1 + Delta / r ** 2 * pr ** 2 + r ** 2 / Lam * L ** 2))
+            hk += L * aa * (1701.0 / 512.0) / r ** 4 / (r ** 3 + aa * aa * (r + 2))
+            assert abs(Heff[0] - hk) < 2e-6, (a, r, Heff[0], hk)
+
+
+def _A_and_Dbar(oracle, add, r):
+    """A = Heff^2 and Dbar = (d^2 Heff^2/dpr^2 / 2) / A^2 at pr = pphi = 0 (nonspinning)."""
+    Heff, H, nu = _Heff(oracle, add, r, 0.0, 0.0)
+    arg = np.array([[r], [0.0], [0.0], [0.0]])
+    hs = oracle.hessian_H(arg, add)[:, :, 0]
+    # H = sqrt(1 + 2 nu (Heff - 1)) / nu (M = 1): H_prpr = Heff_prpr / (nu H) at pr = 0
+    Heff_prpr = hs[2, 2] * nu * H[0]
+    A = Heff[0] ** 2
+    # Heff^2 = A (1 + A Dbar pr^2 + ...) at L = 0  =>  2 Heff Heff_prpr = 2 A^2 Dbar
+    Dbar = Heff[0] * Heff_prpr / (A * A)
+    return A, Dbar
+
+
+def test_A_potential_5p5PN(oracle):
+    eg, pi2 = 0.5772156649015329, math.pi ** 2
+    for q in (1.0, 3.0):
+        add = _nonspinning_add(oracle, q)
+        nu = q / (1 + q) ** 2
+        a4 = (94.0 / 3 - 41 * pi2 / 32) * nu
+        a5c = (-4237.0 / 60 + 2275 * pi2 / 512 + 256 * math.log(2) / 5 + 128 * eg / 5) * nu + (-221.0 / 6 + 41 * pi2 / 32) * nu ** 2
+        a5l = 64.0 * nu / 5
+        a6l = -7004.0 / 105 * nu - 144.0 / 5 * nu ** 2
+        a65 = 13696.0 / 525 * math.pi * nu
+        us = np.array([0.006, 0.01, 0.015, 0.02])
+        R = []
+        for u in us:
+            A, _ = _A_and_Dbar(oracle, add, 1 / u)
+            lu = math.log(u)
+            Apub = 1 - 2 * u + 2 * nu * u ** 3 + a4 * u ** 4 + (a5c + a5l * lu) * u ** 5 + a6l * lu * u ** 6 + a65 * u ** 6.5
+            R.append((A - Apub) / u ** 6)
+        R = np.array(R)
+        # what is left is the constant u^6 coefficient plus O(u ln u): nearly flat in u
+        assert np.ptp(R) < 0.25 * max(1.0, np.abs(R).max()), R
+
+
+def test_Dbar_potential_4PN(oracle):
+    eg, pi2 = 0.5772156649015329, math.pi ** 2
+    for q in (1.0, 4.0):
+        add = _nonspinning_add(oracle, q)
+        nu = q / (1 + q) ** 2
+        d4 = (-533.0 / 45 - 23761 * pi2 / 1536 + 1184 * eg / 15 - 6496 * math.log(2) / 15 + 2916 * math.log(3) / 5) * nu \
+            + (123 * pi2 / 16 - 260) * nu ** 2
+        d4l = 592.0 * nu / 15
+        res = []
+        us = np.array([0.006, 0.018])
+        for u in us:
+            _, D = _A_and_Dbar(oracle, add, 1 / u)
+            Dpub = 1 + 6 * nu * u ** 2 + (52 * nu - 6 * nu ** 2) * u ** 3 + (d4 + d4l * math.log(u)) * u ** 4
+            res.append(D - Dpub)
+        # the residual is the next (u^5 ln u) order: its local power law is ~5, a wrong
+        # 4PN coefficient would make it ~4
+        p = math.log(abs(res[1] / res[0])) / math.log(us[1] / us[0])
+        assert 4.6 < p < 5.6, (p, res)
+
+
+def test_spin_orbit_NLO_gyro_factors(oracle):
+    """Hodd = pphi u^3 [gS (chi1 X1^2 + chi2 X2^2) + gS* nu (chi1 + chi2)] / Dn at pr = 0 with
+    gS = 2 - 5 nu u / 8 + O(u^2), gS* = 3/2 - (9/8 + 3 nu / 4) u + O(u^2)."""
+    q, c1, c2 = 2.5, 0.6, -0.3
+    add = _nonspinning_add(oracle, q, chi1=c1, chi2=c2)
+    X1, X2 = add[0, 0], add[1, 0]
+    nu = X1 * X2
+    wS, wT = c1 * X1 ** 2 + c2 * X2 ** 2, nu * (c1 + c2)
+    L = 1.0
+    res = []
+    for u in (0.004, 0.008):
+        r = 1 / u
+        Hp, _, _ = _Heff(oracle, add, r, 0.0, L)
+        Hm, _, _ = _Heff(oracle, add, r, 0.0, -L)
+        Hodd = 0.5 * (Hp[0] - Hm[0])
+        ap2 = (c1 * X1 + c2 * X2) ** 2
+        Dn = 1 + ap2 * (u * u + 2 * u ** 3)
+        G = Hodd * Dn / (L * u ** 3)
+        lead = 2 * wS + 1.5 * wT
+        nlo = (-5 * nu / 8) * wS + (-(9.0 / 8) - 3 * nu / 4) * wT
+        res.append((G - lead - nlo * u) / u ** 2)
+    assert max(abs(x) for x in res) < 50, res
+
+
+def test_gradient_and_hessian_vs_finite_differences(oracle):
+    rng = np.random.default_rng(3)
+    n = 20
+    q = rng.uniform(1, 8, n)
+    add = oracle.make_additional_args(q / (1 + q), 1 / (1 + q), rng.uniform(-0.9, 0.9, n), rng.uniform(-0.9, 0.9, n))
+    arg = np.array([rng.uniform(4, 40, n), rng.uniform(0, 6, n), rng.uniform(-0.05, 0.05, n), np.sqrt(rng.uniform(4, 40, n))])
+    g = oracle.grad_H(arg, add)
+    hs = oracle.hessian_H(arg, add)
+    H = lambda a: oracle.evaluate_H(a[0], a[1], a[2], a[3], add[0], add[1], add[2], add[3])
+    for j in (0, 2, 3):
+        e = 1e-5 * np.maximum(np.abs(arg[j]), 1e-2)
+        ap, am = arg.copy(), arg.copy()
+        ap[j] += e; am[j] -= e
+        # central differences of H ~ 10 carry ~1e-15 / e of rounding: compare at that level
+        np.testing.assert_allclose(g[j], (H(ap) - H(am)) / (2 * e), rtol=1e-6, atol=3e-8)
+        gp, gm = oracle.grad_H(ap, add), oracle.grad_H(am, add)
+        for i in (0, 2, 3):
+            np.testing.assert_allclose(hs[i, j], (gp[i] - gm[i]) / (2 * e), rtol=1e-5, atol=3e-8)
+    assert np.allclose(hs, hs.transpose(1, 0, 2), rtol=1e-12, atol=1e-15)
+    assert np.all(g[1] == 0.0)
+
+
+# ----------------------------------------------------------------- flux
+def test_flux_matches_pn_expansion(oracle):
+    """Circular nonspinning orbits: F / ((32/5) nu^2 v^10) = 1 + (-1247/336 - 35 nu/12) v^2 + 4 pi v^3 + O(v^4)."""
+    for q in (1.5, 3.0):
+        add = _nonspinning_add(oracle, q)
+        co = oracle.fac_coeffs(add)
+        nu = add[0, 0] * add[1, 0]
+        res = []
+        for r in (200.0, 400.0, 800.0):
+            pphi, pr, om, add2 = oracle.initial_conditions(np.array([r]), add, co)
+            arg = np.array([[r], [0.0], [0.0], [pphi[0]]])
+            F = oracle.rr_force(arg, add2, co)
+            flux = -F[1, 0] * nu * om[0]
+            v = om[0] ** (1 / 3)
+            ratio = flux / (32.0 / 5 * nu ** 2 * v ** 10)
+            pn = 1 + (-1247.0 / 336 - 35 * nu / 12) * v ** 2 + 4 * math.pi * v ** 3
+            res.append((ratio - pn) / v ** 4)
+        assert max(abs(x) for x in res) < 60, res
+
+
+def test_newtonian_prefix_22_closed_form(oracle):
+    from eobgpu_amd.pipeline import newton_prefixes
+    m1, m2 = np.array([0.7, 0.5]), np.array([0.3, 0.5])
+    pf = newton_prefixes(m1, m2)
+    np.testing.assert_allclose(pf, oracle.newton_prefixes(m1, m2), rtol=1e-14, atol=1e-16)
+    nu = m1 * m2
+    y22 = math.sqrt(15.0 / (32 * math.pi))
+    # |prefix_22 Y_2^-2(pi/2)| = 8 sqrt(pi/5) nu  (leading-order h_22 amplitude per unit v^2)
+    np.testing.assert_allclose(np.abs(pf[1]) * y22, 8 * math.sqrt(math.pi / 5) * nu, rtol=1e-14)
+
+
+# ------------------------------------------------------ special functions
+def test_complex_gamma_and_ylm(oracle):
+    for z in (3 - 0.4j, 3 - 2.5j, 1.2 + 0.7j, 7.5 - 3j):
+        np.testing.assert_allclose(oracle.cgamma(z), sps.gamma(z), rtol=2e-14)
+    for l in range(2, 9):
+        for m in range(-l, l + 1):
+            phi = 0.37 * l + 0.11 * m
+            ref = sps.sph_harm_y(l, m, math.pi / 2, phi) if hasattr(sps, "sph_harm_y") else sps.sph_harm(m, l, phi, math.pi / 2)
+            np.testing.assert_allclose(oracle.ylm_equator(l, m, phi), ref, rtol=1e-12, atol=1e-15)
+
+
+# ------------------------------------------------------------ spline, likelihood
+def test_spline_build_matches_scipy_not_a_knot(oracle):
+    rng = np.random.default_rng(9)
+    nbin, nsubs, maxlen = 3, 2, 200
+    lengths = np.array([50, 200, 123], dtype=np.int32)
+    t = np.zeros((maxlen, nbin)); y = np.zeros((maxlen, nsubs * nbin))
+    for b in range(nbin):
+        tk = np.cumsum(rng.uniform(0.1, 2.0, lengths[b]))
+        t[:lengths[b], b] = tk
+        for s in range(nsubs):
+            y[:lengths[b], s * nbin + b] = np.cos(0.3 * tk + s) * np.exp(-0.01 * tk)
+    B, c1, c2, c3 = oracle.interpolate_TD(t, y, lengths, nbin, nsubs)
+    for b in range(nbin):
+        L = lengths[b]
+        for s in range(nsubs):
+            j = s * nbin + b
+            cs = CubicSpline(t[:L, b], y[:L, j], bc_type="not-a-knot")
+            np.testing.assert_allclose(B[:L, j], cs(t[:L, b], 1), rtol=1e-9, atol=1e-12)
+            np.testing.assert_allclose(c1[:L - 1, j], cs.c[2], rtol=1e-9, atol=1e-12)
+            np.testing.assert_allclose(c2[:L - 1, j], cs.c[1], rtol=1e-8, atol=1e-11)
+            np.testing.assert_allclose(c3[:L - 1, j], cs.c[0], rtol=1e-8, atol=1e-11)
+            tn = np.sort(rng.uniform(t[0, b], t[L - 1, b], 300))
+            np.testing.assert_allclose(oracle.spline_eval(tn, t[:L, b], y[:L, j], c1[:L, j], c2[:L, j], c3[:L, j], L), cs(tn),
+                                       rtol=1e-9, atol=1e-12)
+
+
+def test_likelihood_partials_match_numpy(oracle):
+    rng = np.random.default_rng(10)
+    nbin, nchan, F = 2, 2, 2500
+    cn = lambda *s: rng.normal(size=s) + 1j * rng.normal(size=s)
+    hp, hc, data = cn(nbin, F), cn(nbin, F), cn(nchan, F)
+    psd = rng.uniform(0.5, 2, (nchan, F))
+    Fp, Fc, ts = rng.normal(size=(nchan, nbin)), rng.normal(size=(nchan, nbin)), rng.normal(size=(nchan, nbin)) * 1e-3
+    df = 0.5
+    parts = oracle.likelihood_partials(hp, hc, data, psd, Fp, Fc, ts, df)
+    f = np.arange(F) * df
+    for b in range(nbin):
+        for c in range(nchan):
+            h = (Fp[c, b] * hp[b] + Fc[c, b] * hc[b]) * np.exp(-2j * np.pi * f * ts[c, b])
+            ref = np.sum(np.abs(data[c] - h) ** 2 / psd[c])
+            np.testing.assert_allclose(parts[:, b, c].sum().real, ref, rtol=1e-12)
+
+
+# ------------------------------------------------------------ C ABI
+def test_c_abi_exports_every_declared_symbol():
+    from eobgpu_amd import _lib
+    header = open(os.path.join(ROOT, "include", "eobhip.h")).read()
+    declared = set(re.findall(r"\b(eobhip_\w+)\s*\(", header))
+    L = _lib.lib()
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(_lib.exported_symbols()) >= declared
+
+
+def test_c_abi_tableau_matches_oracle(oracle):
+    from eobgpu_amd import _lib
+    out = np.zeros(183)
+    assert _lib.lib().eobhip_dop853_tableau(out.ctypes.data_as(ctypes.c_void_p)) == 0
+    np.testing.assert_array_equal(out, oracle.tableau())
