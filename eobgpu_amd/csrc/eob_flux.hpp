@@ -301,19 +301,20 @@ EOB_HD void flux_setup(ModeCoef* mc, WaveCoef& wc, double m1, double m2, double 
 
 // m^2 Omega^2 |h_lm|^2 for one mode (the flux summand before the 1/(8 pi) factor).
 //   E1 = exp(-4 pi H Omega), Heff = (H^2-1)/(2 eta)+1, vPhi2 = (vPhi)^2
-EOB_HD double flux_mode(const ModeCoef& z, double v, double logv, double H, double Omega, double E1, double Heff,
-                        double pphi, double vPhi2) {
-    const double hk = H * z.m * Omega;
-    const double x = 4.0 * kPi * hk;
-    const double Em = ipow(E1, z.m);
-    double prod = 1.0;
-    const double hk4 = 4.0 * hk * hk;
+// (templated on the scalar so the FP64 operation count can be taken with a counting type)
+template <class T>
+EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T E1, T Heff, T pphi, T vPhi2) {
+    const T hk = H * (double)z.m * Omega;
+    const T x = 4.0 * kPi * hk;
+    const T Em = ipow(E1, z.m);
+    T prod = 1.0;
+    const T hk4 = 4.0 * hk * hk;
 #pragma unroll
-    for (int i = 1; i <= 8; ++i) prod *= (i <= z.l) ? (hk4 + (double)(i * i)) : 1.0;
-    const double T2 = x / (1.0 - Em) * prod * z.inv_lfac2;
-    const double el = kEulerGamma + z.log2m + logv;
+    for (int i = 1; i <= 8; ++i) prod = (i <= z.l) ? prod * (hk4 + (double)(i * i)) : prod;
+    const T T2 = x / (1.0 - Em) * prod * z.inv_lfac2;
+    const T el = kEulerGamma + z.log2m + logv;
     // Horner over v^12 .. v^1
-    double acc = z.c[10];
+    T acc = z.c[10];
     acc = acc * v;                                  // v^11 coefficient is zero
     acc = acc * v + (z.c[9] + z.cl10 * el);
     acc = acc * v + z.c[8];
@@ -325,12 +326,12 @@ EOB_HD double flux_mode(const ModeCoef& z, double v, double logv, double H, doub
     acc = acc * v + z.c[2];
     acc = acc * v + z.c[1];
     acc = acc * v + z.c[0];
-    const double rho = 1.0 + acc * v;
-    const double aux = v * (z.f1 + v * v * z.f3);
-    double rpl = ipow(rho, z.l);
+    const T rho = 1.0 + acc * v;
+    const T aux = v * (z.f1 + v * v * z.f3);
+    T rpl = ipow(rho, z.l);
     rpl = (z.flags & 2) ? aux : rpl + aux;
-    const double S = (z.flags & 1) ? Heff : v * pphi;
-    const double hN2 = z.N2 * ipow(vPhi2, z.lpe);
+    const T S = (z.flags & 1) ? Heff : v * pphi;
+    const T hN2 = z.N2 * ipow(vPhi2, z.lpe);
     return z.m2 * Omega * Omega * T2 * S * S * rpl * rpl * hN2;
 }
 

@@ -96,11 +96,12 @@ EOB_HD HDual dlog(HDual x) {
 }
 
 // x^n for small non-negative integer n by repeated squaring
-EOB_HD double ipow(double x, int n) {
-    double r = 1.0;
+template <class T>
+EOB_HD T ipow(T x, int n) {
+    T r = 1.0;
     while (n > 0) {
-        if (n & 1) r *= x;
-        x *= x;
+        if (n & 1) r = r * x;
+        x = x * x;
         n >>= 1;
     }
     return r;

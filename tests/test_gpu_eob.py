@@ -101,13 +101,27 @@ def test_error_controller_dropin(oracle, dev):
     np.testing.assert_allclose(ht.cpu().numpy(), h_ref, rtol=1e-14)
 
 
+_CACHE = {}
+
+
 def _small_batch(oracle, n=16, seed=1):
-    p = configs.small_batch(n=n, seed=seed)
-    add = oracle.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
-    co = oracle.fac_coeffs(add)
-    pphi0, pr0, om0, add = oracle.initial_conditions(p["r0"], add, co)
-    cond = np.array([p["r0"], np.zeros(n), pr0, pphi0])
-    return add, co, cond
+    key = ("batch", n, seed)
+    if key not in _CACHE:
+        p = configs.small_batch(n=n, seed=seed)
+        add = oracle.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
+        co = oracle.fac_coeffs(add)
+        pphi0, pr0, om0, add = oracle.initial_conditions(p["r0"], add, co)
+        cond = np.array([p["r0"], np.zeros(n), pr0, pphi0])
+        _CACHE[key] = (add, co, cond)
+    return _CACHE[key]
+
+
+def _oracle_fixed(oracle, max_step=8192):
+    key = ("fixed", max_step)
+    if key not in _CACHE:
+        add, co, cond = _small_batch(oracle)
+        _CACHE[key] = oracle.run_trajectory(cond, add, co, max_step=max_step, rtol=1e-10, atol=1e-10, hInit=5.0, fix_step1=True)
+    return _CACHE[key]
 
 
 def test_initial_conditions(oracle, dev):
@@ -141,8 +155,7 @@ def _run(add, cond, dev, max_step, **kw):
 def test_trajectory_fixed_step_parity(oracle, dev, group, mode):
     add, co, cond = _small_batch(oracle)
     max_step = 8192
-    t_r, traj_r, ns_r, si_r = oracle.run_trajectory(cond, add, co, max_step=max_step, rtol=1e-10, atol=1e-10, hInit=5.0,
-                                                    fix_step1=True)
+    t_r, traj_r, ns_r, si_r = _oracle_fixed(oracle, max_step)
     t, traj, ns, si, _ = _run(add, cond, dev, max_step, rtol=1e-10, atol=1e-10, hInit=5.0, fix_step=True, mode=mode, group=group)
     assert np.array_equal(ns, ns_r), (ns, ns_r)
     assert np.array_equal(si, si_r), (si, si_r)
