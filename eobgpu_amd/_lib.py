@@ -46,8 +46,9 @@ _SIGS = {
     "eobhip_bin_setup": [c_vp, c_int, c_int, c_vp, c_vp, c_vp],
     "eobhip_initial_conditions": [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_double, c_vp],
     "eobhip_integrate_scratch_bytes": [c_int],
-    "eobhip_integrate": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double,
-                         c_int, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll_p, c_vp],
+    "eobhip_integrate": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
+                         c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                         c_ll_p, c_vp],
     "eobhip_hlm22_knots": [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp],
 }
 _RESTYPES = {

@@ -42,6 +42,9 @@ struct IntegParams {
     int two_pass;             // 1: run_trajectory; 0: one DOPR853.integrate pass
     int fix_step;             // first (or only) pass at fixed step hInit (the reference: off)
     unsigned long long max_attempts;  // per binary, both passes; ends runaway step-size loops
+    const double* x0;         // optional restart: start time per binary (with step0)
+    const int* step0;         // optional restart: row the initial state occupies
+    int use_stop;             // StoppingCriterion on (the reference driver without one uses stop code 0)
 };
 
 template <int G>
@@ -120,9 +123,7 @@ __device__ __forceinline__ void start_pass(GState<G>& s, int pass, double h0) {
 template <int G>
 __device__ __forceinline__ void begin_binary(const IntegParams& P, GState<G>& s, int b, int gl) {
     s.b = b;
-    s.step_num = 0;
     s.sn1 = 0;
-    s.x = 0.0;
     s.att = 0ull;
     s.fail = 0;
 #pragma unroll
@@ -131,7 +132,15 @@ __device__ __forceinline__ void begin_binary(const IntegParams& P, GState<G>& s,
         s.y[q] = (c < 4) ? P.cond[(size_t)c * P.nbin + b] : 0.0;
     }
     start_pass(s, 1, P.hInit);
-    write_row(P, s, gl);
+    if (P.step0) {
+        // continue an earlier integration from row step0 (its row is already stored)
+        s.step_num = P.step0[b];
+        s.x = P.x0[b];
+    } else {
+        s.step_num = 0;
+        s.x = 0.0;
+        write_row(P, s, gl);
+    }
 }
 
 // relaxed device-coherent load (rows written earlier by this group)
@@ -337,6 +346,7 @@ __device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int 
                 if (r < 6.0 && om < s.om3 && third) code = 5;
             }
             s.om0 = s.om1; s.om1 = s.om2; s.om2 = s.om3; s.om3 = om;
+            if (!P.use_stop) code = 0;
             s.code = code;
             s.stop_flag = (code != 0);
         }

@@ -733,9 +733,10 @@ size_t eobhip_integrate_scratch_bytes(int n) {
 }
 
 int eobhip_integrate(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol, double atol,
-                     double hInit, double step_back, double fine_step, int two_pass, int fix_step, int mode, int group, int block,
-                     int grid_groups, double* dense, int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts,
-                     void* scratch, long long* iterations_out, void* stream) {
+                     double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step, const double* x0, const int* step0,
+                     int use_stop, int mode, int group, int block, int grid_groups, double* dense, int* num_steps, int* stop_info,
+                     int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out, void* stream) {
+    EOBHIP_REQUIRE((x0 == nullptr) == (step0 == nullptr), "x0 and step0 go together");
     EOBHIP_REQUIRE(n >= 0, "n < 0");
     EOBHIP_REQUIRE(max_step >= 2, "max_step must be >= 2");
     EOBHIP_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (persistent) or 1 (plain)");
@@ -759,10 +760,13 @@ int eobhip_integrate(const double* condBound, const void* bindata, const void* m
     P.attempts = attempts;
     P.nbin = n;
     P.max_step = max_step;
-    P.rtol = rtol; P.atol = atol; P.hInit = hInit; P.step_back = step_back; P.fine_step = fine_step; P.tmax = 1e13;
+    P.rtol = rtol; P.atol = atol; P.hInit = hInit; P.step_back = step_back; P.fine_step = fine_step; P.tmax = tmax;
     P.two_pass = two_pass ? 1 : 0;
     P.fix_step = fix_step ? 1 : 0;
     P.max_attempts = 16ull * (unsigned long long)max_step + 4096ull;
+    P.x0 = x0;
+    P.step0 = step0;
+    P.use_stop = use_stop ? 1 : 0;
 
     // carve scratch
     char* base = static_cast<char*>(scratch);

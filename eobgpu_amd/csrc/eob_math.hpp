@@ -21,7 +21,7 @@ constexpr double kEulerGamma = 0.5772156649015329;
 constexpr double kLn2 = 0.6931471805599453;
 constexpr double kLn3 = 1.0986122886681098;
 constexpr double kLn5 = 1.6094379124341003;
-constexpr double kLn7 = 1.9459101090932196;
+constexpr double kLn7 = 1.9459101490553132;
 
 struct Dual {
     double v, d;

@@ -117,13 +117,17 @@ class EOBBatch:
 
     # ------------------------------------------------------------- trajectory
     def integrate(self, rtol=1e-12, atol=1e-12, hInit=0.01, step_back=10.0, fine_step=0.05, two_pass=True, fix_step=False,
-                  mode="persistent", group=64, block=256, grid_groups=0):
-        """Run the batch.  Returns the driver iteration count (plain mode) or 0."""
+                  mode="persistent", group=64, block=256, grid_groups=0, x0=None, step0=None, use_stop=True, tmax=1e13):
+        """Run the batch.  Returns the driver iteration count (plain mode) or 0.
+
+        x0/step0 (device tensors [n]) continue an integration from row step0 of self.dense."""
         if self.cond is None:
             raise RuntimeError("initial state not set")
         it = ctypes.c_longlong(0)
         call("eobhip_integrate", self.cond.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, self.max_step,
-             float(rtol), float(atol), float(hInit), float(step_back), float(fine_step), int(bool(two_pass)), int(bool(fix_step)),
+             float(rtol), float(atol), float(hInit), float(step_back), float(fine_step), float(tmax), int(bool(two_pass)),
+             int(bool(fix_step)),
+             None if x0 is None else x0.data_ptr(), None if step0 is None else step0.data_ptr(), int(bool(use_stop)),
              0 if mode == "persistent" else 1, int(group), int(block), int(grid_groups), self.dense.data_ptr(),
              self.num_steps.data_ptr(), self.stop_info.data_ptr(), self.pass1_steps.data_ptr(), self.attempts.data_ptr(),
              self.scratch.data_ptr(), ctypes.byref(it), current_stream())

@@ -133,15 +133,18 @@ int eobhip_initial_conditions(const double* r0, const void* bindata, const void*
  * step with on-device compaction of the active list (the reference's batching model).
  * two_pass = 1 gives run_trajectory semantics (adaptive pass, walk back step_back,
  * fixed-step pass of fine_step); two_pass = 0 one integrate() pass (fix_step as given).
+ * x0/step0 (both NULL, or both [n]): continue from row step0 at time x0 (integrate()'s
+ * step_num/denseOutput restart, eob/new_Dormand_horiz.py:504-520) with condBound holding that row;
+ * use_stop = 0 runs without the StoppingCriterion (stop code 0);
  * dense [n][max_step][5] (t, r, phi, pr, pphi); num_steps[n] = stored points;
  * scratch: eobhip_integrate_scratch_bytes(n) bytes of device memory.
  * iterations_out (host, may be NULL): driver iterations (plain mode) or 0. */
 size_t eobhip_integrate_scratch_bytes(int n);
 int eobhip_integrate(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol,
-                     double atol, double hInit, double step_back, double fine_step, int two_pass, int fix_step, int mode,
-                     int group, int block, int grid_groups, double* dense, int* num_steps, int* stop_info,
-                     int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out,
-                     void* stream);
+                     double atol, double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step,
+                     const double* x0, const int* step0, int use_stop, int mode, int group, int block, int grid_groups,
+                     double* dense, int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts,
+                     void* scratch, long long* iterations_out, void* stream);
 
 /* (2,2) mode on the trajectory knots of every binary (get_hlms for (2,2)):
  * knots [n][max_step][3] = (phi, Re, Im) of h22 * exp(2 i phi); rows >= num_steps untouched */

@@ -13,8 +13,11 @@ pytestmark = pytest.mark.gpu
 
 
 def _states(rng, n):
-    r = rng.uniform(3.0, 60.0, n)
-    pr = rng.uniform(-0.05, 0.0, n)
+    """Half inspiral-like states (r in [3, 60], small |pr|), half plunge-like
+    (r in [2.3, 6], pr in [-1, 0.2]) so that the pr^6 and pr^8 terms matter."""
+    h = n // 2
+    r = np.concatenate([rng.uniform(3.0, 60.0, h), rng.uniform(2.3, 6.0, n - h)])
+    pr = np.concatenate([rng.uniform(-0.05, 0.0, h), rng.uniform(-1.0, 0.2, n - h)])
     L = np.sqrt(r) * rng.uniform(0.9, 1.3, n)
     return np.array([r, rng.uniform(0, 6.0, n), pr, L])
 
