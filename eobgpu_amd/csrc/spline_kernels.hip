@@ -11,6 +11,7 @@
 
 #include <map>
 #include <mutex>
+#include <tuple>
 #include <utility>
 
 #include "eobhip_common.hpp"
@@ -296,9 +297,41 @@ __global__ __launch_bounds__(kBlock) void k_all_in_one_likelihood(double2* temp_
     }
 }
 
-// hipFFT plans, cached per (N, batch, stream)
+// hipFFT plans, cached per (N, rows per call, stream).  A plan owns a work area,
+// so two streams never share one: work queued on different streams may overlap.
+// Rows go through in chunks of kFftChunk (plus single-row calls for the rest),
+// which bounds the plans per stream to two whatever the batch sizes are.
+constexpr int kFftChunk = 64;
 std::mutex g_plan_mu;
-std::map<std::pair<long long, long long>, hipfftHandle> g_plans;
+std::map<std::tuple<long long, int, void*>, hipfftHandle> g_plans;
+
+int fft_rows(const double* in, double* out, int rows, int N, void* stream) {
+    hipfftHandle plan;
+    const auto key = std::make_tuple((long long)N, rows, stream);
+    auto itp = g_plans.find(key);
+    if (itp == g_plans.end()) {
+        int dims[1] = {N};
+        hipfftResult r = hipfftPlanMany(&plan, 1, dims, nullptr, 1, N, nullptr, 1, N / 2 + 1, HIPFFT_D2Z, rows);
+        if (r != HIPFFT_SUCCESS) {
+            set_error("hipfftPlanMany failed (%d)", (int)r);
+            return EOBHIP_EHIP;
+        }
+        r = hipfftSetStream(plan, as_stream(stream));
+        if (r != HIPFFT_SUCCESS) {
+            set_error("hipfftSetStream failed (%d)", (int)r);
+            return EOBHIP_EHIP;
+        }
+        g_plans[key] = plan;
+    } else {
+        plan = itp->second;
+    }
+    hipfftResult r = hipfftExecD2Z(plan, const_cast<double*>(in), reinterpret_cast<hipfftDoubleComplex*>(out));
+    if (r != HIPFFT_SUCCESS) {
+        set_error("hipfftExecD2Z failed (%d)", (int)r);
+        return EOBHIP_EHIP;
+    }
+    return EOBHIP_OK;
+}
 
 }  // namespace
 
@@ -351,33 +384,17 @@ int eobhip_resample_hplus(const double* dense, const double* coef, const int* nu
 // Batched real-to-complex FFT of n rows of N samples into n rows of N/2+1 bins (no dt factor,
 // as the reference's rfft, eob/seobnrv4phm.py:742-744).
 int eobhip_rfft_batched(const double* in, double* out, int n, int N, void* stream) {
-    EOBHIP_REQUIRE(n >= 1 && N >= 2, "bad sizes");
-    hipfftHandle plan;
-    {
-        std::lock_guard<std::mutex> g(g_plan_mu);
-        auto key = std::make_pair((long long)N, (long long)n);
-        auto itp = g_plans.find(key);
-        if (itp == g_plans.end()) {
-            int dims[1] = {N};
-            hipfftResult r = hipfftPlanMany(&plan, 1, dims, nullptr, 1, N, nullptr, 1, N / 2 + 1, HIPFFT_D2Z, n);
-            if (r != HIPFFT_SUCCESS) {
-                set_error("hipfftPlanMany failed (%d)", (int)r);
-                return EOBHIP_EHIP;
-            }
-            g_plans[key] = plan;
-        } else {
-            plan = itp->second;
-        }
-        hipfftResult r = hipfftSetStream(plan, as_stream(stream));
-        if (r != HIPFFT_SUCCESS) {
-            set_error("hipfftSetStream failed (%d)", (int)r);
-            return EOBHIP_EHIP;
-        }
-        r = hipfftExecD2Z(plan, const_cast<double*>(in), reinterpret_cast<hipfftDoubleComplex*>(out));
-        if (r != HIPFFT_SUCCESS) {
-            set_error("hipfftExecD2Z failed (%d)", (int)r);
-            return EOBHIP_EHIP;
-        }
+    EOBHIP_REQUIRE(n >= 1 && N >= 2 && (N % 2) == 0, "bad sizes (n >= 1, N even)");
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    const size_t F = (size_t)N / 2 + 1;
+    int row = 0;
+    for (; row + kFftChunk <= n; row += kFftChunk) {
+        const int rc = fft_rows(in + (size_t)row * N, out + 2 * (size_t)row * F, kFftChunk, N, stream);
+        if (rc) return rc;
+    }
+    for (; row < n; ++row) {
+        const int rc = fft_rows(in + (size_t)row * N, out + 2 * (size_t)row * F, 1, N, stream);
+        if (rc) return rc;
     }
     return EOBHIP_OK;
 }

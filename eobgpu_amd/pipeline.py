@@ -83,7 +83,10 @@ class EOBBatch:
     """Device-resident tables and trajectory buffers for n binaries."""
 
     def __init__(self, additional_args, max_step=4096, device="cuda", use_hm=1):
-        add = torch.as_tensor(np.ascontiguousarray(additional_args, dtype=np.float64), device=device)
+        if isinstance(additional_args, torch.Tensor):
+            add = additional_args.to(device=device, dtype=torch.float64).contiguous()
+        else:
+            add = torch.as_tensor(np.ascontiguousarray(additional_args, dtype=np.float64), device=device)
         self.n = int(add.shape[1])
         self.device = device
         self.max_step = int(max_step)

@@ -354,7 +354,9 @@ long orc_integrate(const double* condBound, const double* add, const double* coe
     }
 
     std::vector<double> x(n), h(n), y((size_t)4 * n), errOld(n, 1e-4), hInitOrig(n, hInit);
-    std::vector<char> prevReject(n, 0), active(n, 1), stop(n, 1);
+    std::vector<char> prevReject(n, 0), active(n, 1), stop(n, 1), failed(n, 0);
+    std::vector<long long> attempts(n, 0);
+    const long long max_attempts = 16LL * max_step + 4096;
     std::vector<double> old_omegas((size_t)max_step * n, 0.0);
     for (int i = 0; i < n; ++i) {
         if (!restart) {
@@ -388,6 +390,11 @@ long orc_integrate(const double* condBound, const double* add, const double* coe
             prevReject[i] = pr;
             if (fix_step && !ok) hInitOrig[i] /= 2;
             if (fix_step) h[i] = hInitOrig[i];
+            // A non-finite error or step (or a step halved to 0) would make the
+            // reference's loop run forever; the lane ends with stop code -1
+            // (-2 after max_attempts attempts), as the integrator kernel does.
+            if (!(err >= 0.0) || !(std::fabs(h[i]) < 1e300) || h[i] == 0.0) failed[i] = 1;
+            else if (++attempts[i] >= max_attempts) failed[i] = 2;
             if (ok) {
                 for (int j = 0; j < 4; ++j) y[j * n + i] = yn[j];
                 x[i] = x[i] + hOld;
@@ -459,7 +466,8 @@ long orc_integrate(const double* condBound, const double* add, const double* coe
         bool any = false;
         for (int i = 0; i < n; ++i) {
             if (!active[i]) continue;
-            if (x[i] >= tmax || step_num[i] >= max_step - 1 || stop[i]) active[i] = 0;
+            if (failed[i]) stop_info[i] = -failed[i];
+            if (x[i] >= tmax || step_num[i] >= max_step - 1 || stop[i] || failed[i]) active[i] = 0;
             any |= active[i] != 0;
         }
         ++iter;
@@ -481,7 +489,10 @@ void orc_run_trajectory(const double* condBound, const double* add, const double
     std::memset(traj, 0, sizeof(double) * (size_t)max_step * 4 * n);
     orc_integrate(condBound, add, coeffs, n, hInit, max_step, 1e13, fix_step1, rtol, atol, sn.data(), traj, t_out, 0, stop_info,
                   batch_coupled);
+    std::vector<int> keep;                  // lanes that go on to the fine pass
     for (int i = 0; i < n; ++i) {
+        if (stop_info[i] < 0) continue;     // ended by the non-finite guard: no second pass
+        keep.push_back(i);
         // t_stop is t[num_steps - 1]: the reference finds the last non-zero row of
         // t[1:] and then indexes the unshifted t with it (eob/seobnrv4phm.py:2095-2097)
         int last = sn[i] - 1;
@@ -500,7 +511,39 @@ void orc_run_trajectory(const double* condBound, const double* add, const double
         }
         sn[i] = best - 1;
     }
-    orc_integrate(condBound, add, coeffs, n, fine_step, max_step, 1e13, 1, rtol, atol, sn.data(), traj, t_out, 1, stop_info, batch_coupled);
+    if ((int)keep.size() == n) {
+        orc_integrate(condBound, add, coeffs, n, fine_step, max_step, 1e13, 1, rtol, atol, sn.data(), traj, t_out, 1, stop_info,
+                      batch_coupled);
+    } else if (!keep.empty()) {
+        // run the fine pass on the kept lanes only (gather, integrate, scatter)
+        const int m = (int)keep.size();
+        std::vector<double> cb((size_t)4 * m), ad((size_t)NADD * m), to((size_t)max_step * m), tr((size_t)max_step * 4 * m),
+            co((size_t)m * sizeof(FacCoeffs) / sizeof(double));
+        std::vector<int> sm(m), sim(m);
+        const FacCoeffs* hc = reinterpret_cast<const FacCoeffs*>(coeffs);
+        for (int q = 0; q < m; ++q) {
+            const int i = keep[q];
+            for (int j = 0; j < 4; ++j) cb[(size_t)j * m + q] = condBound[(size_t)j * n + i];
+            for (int j = 0; j < NADD; ++j) ad[(size_t)j * m + q] = add[(size_t)j * n + i];
+            for (int s = 0; s < max_step; ++s) {
+                to[(size_t)s * m + q] = t_out[(size_t)s * n + i];
+                for (int j = 0; j < 4; ++j) tr[((size_t)s * 4 + j) * m + q] = traj[((size_t)s * 4 + j) * n + i];
+            }
+            reinterpret_cast<FacCoeffs*>(co.data())[q] = hc[i];
+            sm[q] = sn[i];
+        }
+        orc_integrate(cb.data(), ad.data(), co.data(), m, fine_step, max_step, 1e13, 1, rtol, atol, sm.data(), tr.data(), to.data(), 1,
+                      sim.data(), batch_coupled);
+        for (int q = 0; q < m; ++q) {
+            const int i = keep[q];
+            for (int s = 0; s < max_step; ++s) {
+                t_out[(size_t)s * n + i] = to[(size_t)s * m + q];
+                for (int j = 0; j < 4; ++j) traj[((size_t)s * 4 + j) * n + i] = tr[((size_t)s * 4 + j) * m + q];
+            }
+            sn[i] = sm[q];
+            stop_info[i] = sim[q];
+        }
+    }
     for (int i = 0; i < n; ++i) num_steps[i] = sn[i] + 1;
 }
 

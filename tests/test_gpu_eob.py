@@ -41,7 +41,10 @@ def test_hamiltonian_value_grad_hessian(oracle, dev):
     cols = [c.contiguous() for c in cols]
     pyEOB.evaluate_Ham_align_AD(out, *cols, 0.0, 0.0, 0.0, 0.0, n)
     H_ref = oracle.evaluate_H(arg[0], arg[1], arg[2], arg[3], add[0], add[1], add[2], add[3])
-    np.testing.assert_allclose(out.cpu().numpy(), H_ref, rtol=1e-13, atol=0)
+    # In the plunge states cancellation amplifies rounding: the oracle itself built
+    # with FMA contraction (as hipcc builds the kernels) moves H by 1.2e-13 and the
+    # RHS by up to 1.0e-10 relative on these inputs, so the bounds sit above that.
+    np.testing.assert_allclose(out.cpu().numpy(), H_ref, rtol=5e-13, atol=0)
 
     g = torch.zeros((4, n), dtype=torch.float64, device=dev)
     pyEOB.grad_Ham_align_AD(t_arg, g, t_add, n)
@@ -72,7 +75,7 @@ def test_ode_rhs_dropin(oracle, dev):
     k = torch.zeros((4, n), dtype=torch.float64, device=dev)
     idx = torch.arange(n, dtype=torch.int32, device=dev)
     ode.ODE_Ham_align_AD(torch.zeros(n, dtype=torch.float64, device=dev), T(arg), k, T(add), n, idx)
-    np.testing.assert_allclose(k.cpu().numpy(), k_ref, rtol=1e-11, atol=1e-18)
+    np.testing.assert_allclose(k.cpu().numpy(), k_ref, rtol=5e-10, atol=1e-18)
     ode.deallocate_information()
 
 

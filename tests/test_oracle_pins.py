@@ -6,15 +6,16 @@
   - the Kerr test-mass limit of the Hamiltonian;
   - published post-Newtonian EOB potentials: A(u) through 5.5PN
     (Bini-Damour 2013; Damour-Jaranowski-Schaefer 2015: 4PN a5, the 5PN log and
-    the 5.5PN u^(13/2) terms), Dbar(u) through 4PN, the NLO spin-orbit
-    gyro-gravitomagnetic factors (Damour-Jaranowski-Schaefer 2008);
+    the 5.5PN u^(13/2) terms), Dbar(u) through 4PN, Q through 4PN (its u^2 pr^4,
+    u^3 pr^4 and u^2 pr^6 terms), the NLO spin-orbit gyro-gravitomagnetic
+    factors (Damour-Jaranowski-Schaefer 2008);
   - finite differences of H for the AD gradient and Hessian;
   - the PN energy flux (quadrupole normalization + 1PN + tail) for the factorized flux;
   - scipy CubicSpline (not-a-knot) for the spline build, numpy for the likelihood,
     scipy.special for Gamma(z) and Y_lm.
-Coefficients beyond those orders (e.g. the N3LO spin-orbit tables, the pr^6, pr^8
-Q terms, the rho_lm tables past the orders checked by the flux expansion) have
-no independent check here: parity unpinned.
+Coefficients beyond those orders (e.g. the N3LO spin-orbit tables, the 5PN Q
+terms (u^3 pr^6, the pr^8 terms, u^4 pr^4), the rho_lm tables past the orders
+checked by the flux expansion) have no independent check here: parity unpinned.
 """
 import ctypes
 import math
@@ -163,6 +164,38 @@ def test_Dbar_potential_4PN(oracle):
         # 4PN coefficient would make it ~4
         p = math.log(abs(res[1] / res[0])) / math.log(us[1] / us[0])
         assert 4.6 < p < 5.6, (p, res)
+
+
+def test_Q_potential_4PN(oracle):
+    """Q at 3PN and 4PN (Damour-Jaranowski-Schaefer 2000, 2015):
+    Q = 2 nu (4 - 3 nu) u^2 pr^4
+        + [(-5308/15 + 496256/45 ln2 - 33048/5 ln3) nu - 83 nu^2 + 10 nu^3] u^3 pr^4
+        + [(-827/3 - 2358912/25 ln2 + 1399437/50 ln3 + 390625/18 ln5) nu - 27/5 nu^2 + 6 nu^3] u^2 pr^6 + ...
+    At L = 0 (nonspinning) Heff^2 / A - 1 = A Dbar pr^2 + Q is a polynomial in pr^2;
+    its pr^4 and pr^6 coefficients are fitted at several u and then expanded in u."""
+    l2, l3, l5 = math.log(2), math.log(3), math.log(5)
+    us = np.array([0.002, 0.004, 0.007, 0.01, 0.014, 0.02, 0.03])
+    prs = np.array([0.2, 0.35, 0.5, 0.65, 0.8, 1.0])
+    for q in (1.0, 3.0):
+        add = _nonspinning_add(oracle, q)
+        nu = q / (1 + q) ** 2
+        c4, c6 = [], []
+        for u in us:
+            A = _Heff(oracle, add, 1 / u, 0.0, 0.0)[0][0] ** 2
+            f = np.array([_Heff(oracle, add, 1 / u, p, 0.0)[0][0] ** 2 / A - 1 for p in prs])
+            c = np.linalg.lstsq(np.stack([prs ** 2, prs ** 4, prs ** 6, prs ** 8], 1), f, rcond=None)[0]
+            c4.append(c[1])
+            c6.append(c[2])
+        # pr^4: u^2 (q2 + q3 u + (q4 + q4l ln u) u^2 + q92 u^2.5); pr^6: u^2 (p2 + p3 u + p72 u^1.5)
+        a4 = np.linalg.lstsq(np.stack([np.ones_like(us), us, us ** 2, us ** 2 * np.log(us), us ** 2.5], 1),
+                             np.array(c4) / us ** 2, rcond=None)[0]
+        a6 = np.linalg.lstsq(np.stack([np.ones_like(us), us, us ** 1.5], 1), np.array(c6) / us ** 2, rcond=None)[0]
+        q42 = 2 * nu * (4 - 3 * nu)
+        q43 = (-5308.0 / 15 + 496256.0 / 45 * l2 - 33048.0 / 5 * l3) * nu - 83 * nu ** 2 + 10 * nu ** 3
+        q62 = (-827.0 / 3 - 2358912.0 / 25 * l2 + 1399437.0 / 50 * l3 + 390625.0 / 18 * l5) * nu - 27.0 / 5 * nu ** 2 + 6 * nu ** 3
+        assert a4[0] == pytest.approx(q42, rel=1e-7)
+        assert a4[1] == pytest.approx(q43, rel=1e-5)
+        assert a6[0] == pytest.approx(q62, rel=1e-6)
 
 
 def test_spin_orbit_NLO_gyro_factors(oracle):
