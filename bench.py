@@ -4,13 +4,17 @@ uneven aligned-spin binaries on one MI355X.
 One step = one full pass of the batched hot path over the batch:
   per-binary tables -> circular initial state at r0 -> two-pass DOP853
   trajectory (persistent work-stealing kernel) -> (2,2) mode on the knots ->
-  not-a-knot splines -> uniform resample of h+ (N = 2^20 samples, dt = 1/4096 s)
-  -> row FFT (hipFFT) -> frequency-domain likelihood against synthetic data
-  (injection of binary 0 + Gaussian noise, analytic PSD S(f) = (f/f0)^-4 + 1 + (f/f0)^2).
-value = n_binaries / step time.  The same pass with the reference's batching model
-(one launch per attempted step, lane per binary, host-synchronised compaction)
-is timed beside it as plain_batch, and the oracle (CPU restatement) is timed on a
-bounded sample of the same binaries as cpu_baseline.
+  not-a-knot splines -> h+ on the uniform grid (N = 2^20 samples, dt = 1/4096 s)
+  -> frequency-domain likelihood against synthetic data (injection of binary 0
+  + Gaussian noise, analytic PSD S(f) = (f/f0)^-4 + 1 + (f/f0)^2), evaluated
+  from each template's support (one M-point FFT, M >= 2 x support) instead of
+  an N-point FFT per template; the full-length path is timed beside it and the
+  two log-likelihoods are compared.
+value = n_binaries / step time.  The reference's batching model (one launch per
+attempted step, lane per binary, host-synchronised compaction) is timed beside
+it as plain_batch, the continuous-batching server on Poisson arrivals as
+served, and the oracle (CPU restatement) on a bounded sample of the same
+binaries as cpu_baseline.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -30,7 +34,7 @@ sys.path.insert(0, ROOT)
 # FP64 operations per attempted DOP853 step (12 right-hand sides + stages +
 # error norm + controller), counted by eobgpu_amd/csrc/flop_count.cpp on the
 # library's own Hamiltonian and flux code.
-STEP_FLOPS = 55679
+STEP_FLOPS = 50099
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -47,9 +51,16 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-binaries", type=int, default=48)
+    ap.add_argument("--cpu-binaries", type=int, default=200)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--serve-streams", type=int, default=8)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
+                    help="uneven: the headline 1024-binary pipeline; spline: 512 synthetic (2,2) modes with 5k-60k knots "
+                         "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator")
     return ap.parse_args()
 
 
@@ -76,30 +87,34 @@ class Pipeline:
         self.partial = torch.empty(int(P.lib().eobhip_fd_likelihood_partial_doubles(n, self.F)), dtype=torch.float64, device=dev)
         self.amp = np.full(n, 1.0)
         self.data = None
+        self.lik = None
 
-    def run(self, mode="persistent", group=64, events=None):
+    def run(self, mode="persistent", group=64, events=None, direct=False, block=256, grid_groups=0):
         P, b = self.P, self.b
         rec = (lambda k: events[k].record()) if events is not None else (lambda k: None)
         rec("t0")
         P.call("eobhip_bin_setup", b.add.data_ptr(), b.n, 1, b.bindata.data_ptr(), b.modes.data_ptr(), P.current_stream())
         b.initial_conditions(self.params["r0"])
         rec("t1")
-        b.integrate(rtol=self.rtol, atol=self.rtol, two_pass=True, mode=mode, group=group)
+        b.integrate(rtol=self.rtol, atol=self.rtol, two_pass=True, mode=mode, group=group, block=block, grid_groups=grid_groups)
         rec("t2")
         b.hlm22_knots(out=getattr(b, "knots", None))
         b.build_splines()
-        b.resample_hplus(self.M, self.dt, self.N, amp=self.amp, out=self.hp)
         rec("t3")
-        P.rfft_batched(self.hp, out=self.ht)
+        if direct or self.lik is None:
+            # full-length path: h+ on N samples, N-point FFT, sum over N/2+1 bins
+            b.resample_hplus(self.M, self.dt, self.N, amp=self.amp, out=self.hp)
+            P.rfft_batched(self.hp, out=self.ht)
+            if self.data is not None:
+                P.fd_likelihood(self.ht, self.data, self.df, partial=self.partial, out=self.logl)
+        else:
+            self.lik(b, self.M, self.dt, amp=self.amp, out=self.logl)
         rec("t4")
-        if self.data is not None:
-            P.fd_likelihood(self.ht, self.data, self.df, partial=self.partial, out=self.logl)
-        rec("t5")
 
     def make_data(self, snr=20.0, seed=7):
         """d = injection (binary 0, scaled to the target SNR) + noise with the analytic PSD."""
         torch = self.torch
-        self.run()
+        self.run(direct=True)
         torch.cuda.synchronize()
         f = np.arange(self.F) * self.df
         S = self.P.analytic_psd(f)
@@ -112,19 +127,20 @@ class Pipeline:
         sig = np.sqrt(np.where(f > 0, S, 0.0) / (4 * self.df))
         noise = sig * (rng.normal(size=self.F) + 1j * rng.normal(size=self.F)) / np.sqrt(2)
         self.data = torch.tensor(h0 * scale + noise, device=self.b.device)
+        self.lik = self.P.SupportLikelihood(self.data, self.N, self.df)
 
 
-def time_pipeline(pipe, mode, group, steps, warmup):
+def time_pipeline(pipe, mode, group, steps, warmup, direct=False, block=256, grid_groups=0):
     torch = pipe.torch
     for _ in range(warmup):
-        pipe.run(mode=mode, group=group)
+        pipe.run(mode=mode, group=group, direct=direct, block=block, grid_groups=grid_groups)
     torch.cuda.synchronize()
-    keys = ["t0", "t1", "t2", "t3", "t4", "t5"]
-    stage = {k: 0.0 for k in ("setup_ic", "trajectory", "waveform", "fft", "likelihood")}
+    keys = ["t0", "t1", "t2", "t3", "t4"]
+    stage = {k: 0.0 for k in ("setup_ic", "trajectory", "knots_splines", "likelihood")}
     t0 = time.perf_counter()
     for _ in range(steps):
         ev = {k: torch.cuda.Event(enable_timing=True) for k in keys}
-        pipe.run(mode=mode, group=group, events=ev)
+        pipe.run(mode=mode, group=group, events=ev, direct=direct, block=block, grid_groups=grid_groups)
         torch.cuda.synchronize()
         for name, (a, c) in zip(stage, zip(keys[:-1], keys[1:])):
             stage[name] += ev[a].elapsed_time(ev[c])
@@ -149,6 +165,168 @@ def cpu_baseline(pipe, nb):
     return nb / el, el
 
 
+def serve_section(args):
+    """Poisson arrivals (200/s) of proposal batches of U{1..256} binaries drawn as the
+    16-binary small workload (q ~ U[1,5], M = 30, chi ~ U[-0.5,0.5], r0 = 20), each a
+    full waveform + likelihood at N = 2^12, dt = 1/2048 s; continuous batching vs one
+    request at a time with the one-launch-per-step integrator."""
+    import torch
+    from eobgpu_amd import configs, pipeline as P
+    from eobgpu_amd.serving import Request, Server, summarize
+    N, dt = 1 << 12, 1.0 / 2048
+    F = N // 2 + 1
+    df = 1.0 / (N * dt)
+    inj = configs.small_batch(n=1, seed=11)
+    b = P.EOBBatch(P.make_additional_args(inj["m1"], inj["m2"], inj["chi1"], inj["chi2"]), max_step=2048)
+    b.initial_conditions(inj["r0"])
+    b.integrate(rtol=1e-10, atol=1e-10, two_pass=True)
+    b.hlm22_knots()
+    b.build_splines()
+    h0 = P.rfft_batched(b.resample_hplus(inj["m1"] + inj["m2"], dt, N))[0].cpu().numpy()
+    f = np.arange(F) * df
+    S = P.analytic_psd(f)
+    rng = np.random.default_rng(17)
+    sig = np.sqrt(np.where(f > 0, S, 0.0) / (4 * df))
+    data = h0 + sig * (rng.normal(size=F) + 1j * rng.normal(size=F)) / np.sqrt(2)
+    reqs = [Request(i, t, p) for i, (t, p) in enumerate(configs.served_requests(duration_s=args.serve_seconds))]
+    srv = Server(data, df=df, dt=dt, N=N, capacity=8192, max_step=2048, streams=args.serve_streams)
+    srv.run([Request(-1 - i, 0.0, configs.small_batch(n=k, seed=900 + i)) for i, k in enumerate((64, 1, 64, 1))])
+    out = {"requests": len(reqs), "N": N, "dt": dt, "streams": args.serve_streams}
+    for policy in ("continuous", "plain"):
+        res = srv.run(reqs, policy=policy)
+        out[policy] = summarize(res)
+        print(f"[bench] served ({policy}): {json.dumps(out[policy])}", file=sys.stderr, flush=True)
+    return out
+
+
+def workload_spline(args):
+    """512 binaries' (2,2) modes on 5k-60k non-uniform knots (chirp spacing
+    t_k = T (1 - (1 - u_k)^(3/8)), u_k sorted uniform), splined and resampled to
+    2^20 samples at 1/4096 s, likelihood against injection + noise (analytic PSD)."""
+    import torch
+    from eobgpu_amd import pipeline as P
+    n, N, dt = 512, 1 << args.log2N, 1.0 / 4096
+    F, df = N // 2 + 1, 1.0 / (N * dt)
+    Msun = 30.0
+    rng = np.random.default_rng(3)
+    K = rng.integers(5000, 60001, n)
+    kmax = int(K.max())
+    span = N * dt / (Msun * P.MTSUN_SI)                 # the data span in units of M
+    T = span * rng.uniform(0.05, 0.9, n)
+    b = P.EOBBatch(P.make_additional_args(np.full(n, 0.6), np.full(n, 0.4), np.zeros(n), np.zeros(n)), max_step=kmax)
+    dense = np.zeros((n, kmax, 5))
+    knots = np.zeros((n, kmax, 3))
+    for i in range(n):
+        u = np.sort(rng.uniform(0.0, 1.0, K[i]))
+        u[0], u[-1] = 0.0, 1.0 - 1e-6
+        t = T[i] * (1.0 - (1.0 - u) ** 0.375)
+        x = 1.0 - t / T[i]
+        f0 = 20.0 * Msun * P.MTSUN_SI                     # 20 Hz in units of 1/M
+        phi = np.pi * f0 * T[i] * 1.6 * (1.0 - x ** 0.625)  # orbital phase; GW frequency 20 Hz x^(-3/8)
+        dense[i, :K[i], 0] = t
+        knots[i, :K[i], 0] = phi
+        knots[i, :K[i], 1] = 1e-21 * x ** -0.25
+        knots[i, :K[i], 2] = 0.0
+    b.dense.copy_(torch.from_numpy(dense))
+    b.knots = torch.from_numpy(knots).to(b.device)
+    b.num_steps.copy_(torch.from_numpy(K.astype(np.int32)))
+    del dense, knots
+    b.build_splines()
+    hp = b.resample_hplus(np.full(n, Msun), dt, N)
+    h0 = P.rfft_batched(hp[:1].contiguous())[0].cpu().numpy()
+    del hp
+    f = np.arange(F) * df
+    S = P.analytic_psd(f)
+    w = np.where(f > 0, 1.0 / S, 0.0)
+    rho = np.sqrt(4 * df * np.sum(np.abs(h0) ** 2 * w))
+    sig = np.sqrt(np.where(f > 0, S, 0.0) / (4 * df))
+    data = h0 * (20.0 / rho) + sig * (rng.normal(size=F) + 1j * rng.normal(size=F)) / np.sqrt(2)
+    lik = P.SupportLikelihood(data, N, df)
+    logl = torch.empty(n, dtype=torch.float64, device=b.device)
+
+    def step():
+        b.build_splines()
+        lik(b, np.full(n, Msun), dt, out=logl)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    t0 = time.perf_counter()
+    spl = lk = 0.0
+    for _ in range(args.steps):
+        ev[0].record()
+        b.build_splines()
+        ev[1].record()
+        lik(b, np.full(n, Msun), dt, out=logl)
+        ev[2].record()
+        torch.cuda.synchronize()
+        spl += ev[0].elapsed_time(ev[1])
+        lk += ev[1].elapsed_time(ev[2])
+    wall = (time.perf_counter() - t0) / args.steps
+    spl /= args.steps
+    lk /= args.steps
+    knots_total = int(K.sum())
+    # spline build: 8 (t) + 24 (three y) read + 96 (coefficients) written per knot; support likelihood:
+    # h on M_b samples written and read by the FFT, M_b/2+1 bins read back
+    mb = (1 << lik.mlog2.cpu().numpy().astype(np.int64))
+    lik_bytes = float(np.sum(8 * mb * 2 + 16 * (mb // 2 + 1)))
+    out = {"metric": "likelihood evals/s (512 synthetic (2,2) modes, 5k-60k knots, N=2^20)", "value": n / wall,
+           "unit": "likelihood evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "512 synthetic chirp (2,2) modes, K ~ U{5000..60000} knots, M = 30 Msun, N = 2^20, dt = 1/4096 s",
+                      "knots_total": knots_total},
+           "stages_ms": {"spline_build": spl, "likelihood": lk},
+           "roofline": {"spline_build": {"bound": "hbm", "achieved": knots_total * 128 / (spl * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                                         "unit": "GB/s", "frac": knots_total * 128 / (spl * 1e-3) / 1e9 / PEAK_HBM_GBS},
+                        "likelihood": {"bound": "hbm", "achieved": lik_bytes / (lk * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
+                                       "unit": "GB/s", "frac": lik_bytes / (lk * 1e-3) / 1e9 / PEAK_HBM_GBS}},
+           "logl_finite": bool(torch.isfinite(logl).all().item())}
+    print(json.dumps(out))
+
+
+def workload_capacity(args):
+    """16384 binaries, r0 ~ U[30, 60] M: the two-pass trajectory of the whole
+    batch with dense per-step output kept on the device."""
+    import torch
+    from eobgpu_amd import autotune, configs, pipeline as P
+    p = configs.capacity_batch()
+    n = p["m1"].size
+    add = P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
+    max_step = max(args.max_step, 8192)
+    tuned = autotune.tune(add, p["r0"], max_step=max_step, sample=2048, rtol=args.rtol, atol=args.rtol)
+    b = P.EOBBatch(add, max_step=max_step)
+    b.initial_conditions(p["r0"])
+    cond = b.cond.clone()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    times = []
+    for it in range(args.warmup + args.steps):
+        b.cond = cond
+        ev[0].record()
+        b.integrate(rtol=args.rtol, atol=args.rtol, two_pass=True, group=tuned["group"], block=tuned["block"],
+                    grid_groups=tuned["grid_groups"])
+        ev[1].record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(ev[0].elapsed_time(ev[1]))
+    ms = float(np.mean(times))
+    att = float(b.attempts.sum().item())
+    ns = b.num_steps.cpu().numpy()
+    si = b.stop_info.cpu().numpy()
+    ach = att * STEP_FLOPS / (ms * 1e-3) / 1e12
+    out = {"metric": "binaries/s (16384-binary capacity batch, trajectory)", "value": n / (ms * 1e-3), "unit": "binaries/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "16384 binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[30,60] M, two-pass DOP853 1e-10",
+                      "max_step": max_step, "dense_bytes": int(b.dense.numel() * 8), "group": tuned["group"],
+                      "block": tuned["block"], "grid_groups": tuned["grid_groups"]},
+           "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
+           "attempted_steps_total": int(att), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
+           "roofline": {"bound": "fp64-vector", "kernel": "k_integrate_persistent", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS}}
+    print(json.dumps(out))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -157,11 +335,23 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.init()
+    if args.workload == "spline":
+        return workload_spline(args)
+    if args.workload == "capacity":
+        return workload_capacity(args)
     pipe = Pipeline(args.n, args.log2N, args.max_step, args.rtol)
     pipe.make_data()
     print(f"[bench] data ready, n={args.n} N=2^{args.log2N}", file=sys.stderr, flush=True)
 
-    wall, stages = time_pipeline(pipe, "persistent", args.group, args.steps, args.warmup)
+    tuned = None
+    group, block, grid = args.group, 256, 0
+    if not args.no_tune:
+        from eobgpu_amd import autotune
+        tuned = autotune.tune(pipe.add, pipe.params["r0"], max_step=args.max_step, rtol=args.rtol, atol=args.rtol)
+        group, block, grid = tuned["group"], tuned["block"], tuned["grid_groups"]
+        print(f"[bench] autotune: group {group} block {block} grid_groups {grid} ({len(tuned['table'])} shapes)",
+              file=sys.stderr, flush=True)
+    wall, stages = time_pipeline(pipe, "persistent", group, args.steps, args.warmup, block=block, grid_groups=grid)
     b = pipe.b
     ns = b.num_steps.cpu().numpy()
     att = b.attempts.cpu().numpy()
@@ -171,12 +361,29 @@ def main():
     print(f"[bench] persistent: {wall*1e3:.2f} ms/step stages {json.dumps({k: round(v, 3) for k, v in stages.items()})}",
           file=sys.stderr, flush=True)
 
+    # the full-length likelihood path (N-sample h+, N-point FFT, N/2+1-bin sum) on
+    # the same batch, for its time and for the agreement of the two results
+    logl_support = logl.copy()
+    wall_d, stages_d = time_pipeline(pipe, "persistent", group, 1, 0, direct=True, block=block, grid_groups=grid)
+    logl_direct = pipe.logl.cpu().numpy()
+    full_length = {"ms_per_step": wall_d * 1e3, "likelihood_ms": stages_d["likelihood"],
+                   "max_rel_diff_logl": float(np.max(np.abs(logl_support - logl_direct) / np.abs(logl_direct)))}
+    print(f"[bench] full-length likelihood path: {json.dumps(full_length)}", file=sys.stderr, flush=True)
+
     plain = None
     if not args.no_plain:
-        wall_p, stages_p = time_pipeline(pipe, "plain", args.plain_group, args.plain_steps, 0)
-        plain = {"value": args.n / wall_p, "ms_per_step": wall_p * 1e3, "group": args.plain_group,
-                 "trajectory_ms": stages_p["trajectory"], "speedup_of_persistent": wall_p / wall}
-        print(f"[bench] plain: {wall_p*1e3:.2f} ms/step", file=sys.stderr, flush=True)
+        # the reference's batching model: one launch per attempted step, one lane per
+        # binary; and the same per-step launches with this build's 64-lane groups
+        plain = {}
+        for name, grp in (("lane_per_binary", args.plain_group), ("group64", 64)):
+            wall_p, stages_p = time_pipeline(pipe, "plain", grp, args.plain_steps, 0)
+            plain[name] = {"value": args.n / wall_p, "ms_per_step": wall_p * 1e3, "group": grp,
+                           "trajectory_ms": stages_p["trajectory"], "speedup_of_persistent": wall_p / wall}
+            print(f"[bench] plain ({name}): {wall_p*1e3:.2f} ms/step", file=sys.stderr, flush=True)
+
+    served = None
+    if not args.no_serve:
+        served = serve_section(args)
 
     # roofline of the dominant kernel
     dom = max(stages, key=stages.get)
@@ -187,9 +394,9 @@ def main():
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                 "algorithmic": f"{STEP_FLOPS} FP64 ops per attempted step x {int(total_attempts)} attempted steps"}
     else:
-        nbytes = {"fft": 2 * args.n * pipe.N * 8 + args.n * pipe.F * 16,
-                  "waveform": args.n * pipe.N * 8,
-                  "likelihood": args.n * pipe.F * 16 + pipe.F * 16}.get(dom, 0)
+        # support path: h on M_b samples written and read by the FFT, the M_b/2+1 bins read back
+        mb = (1 << pipe.lik.mlog2.cpu().numpy().astype(np.int64)) if pipe.lik is not None else np.zeros(1)
+        nbytes = {"likelihood": float(np.sum(8 * mb * 2 + 16 * (mb // 2 + 1)))}.get(dom, 0)
         achieved = nbytes / (stages[dom] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": achieved / PEAK_HBM_GBS, "traffic": None}
@@ -205,12 +412,14 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[15,40] M; "
-                               "two-pass DOP853 rtol=atol=1e-10; (2,2) resampled to 2^20 samples at 1/4096 s; FD likelihood",
-                   "global_batch": args.n, "seq_len": pipe.N, "parallelism": "none (1 GPU)", "group_lanes": args.group},
+                               "two-pass DOP853 rtol=atol=1e-10; (2,2) on 2^20 samples at 1/4096 s; FD likelihood (support-length transforms)",
+                   "global_batch": args.n, "seq_len": pipe.N, "parallelism": "none (1 GPU)", "group_lanes": group,
+                   "block": block, "grid_groups": grid},
+        "autotune": None if tuned is None else {k: tuned[k] for k in ("group", "block", "grid_groups", "time_s", "table")},
         "stages_ms": stages, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
         "attempted_steps_total": int(total_attempts), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
         "logl_finite": bool(np.all(np.isfinite(logl))),
-        "plain_batch": plain, "roofline": roof, "cpu_baseline": cpu,
+        "full_length_likelihood": full_length, "plain_batch": plain, "served": served, "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -68,6 +68,17 @@ _SIGS.update({
     "eobhip_fd_likelihood_partial_doubles": [c_int, c_int],
 })
 _RESTYPES["eobhip_fd_likelihood_partial_doubles"] = c_size
+_SIGS.update({
+    "eobhip_lik_bank_doubles": [c_int, c_int],
+    "eobhip_lik_setup_scratch_bytes": [c_int],
+    "eobhip_lik_setup": [c_vp, c_int, c_double, c_double, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_lik_sizes": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp],
+    "eobhip_lik_partial_blocks": [c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+    "eobhip_lik_resample": [c_vp] * 8 + [c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_lik_finish": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_double, c_vp, c_vp, c_vp],
+})
+_RESTYPES["eobhip_lik_bank_doubles"] = c_size
+_RESTYPES["eobhip_lik_setup_scratch_bytes"] = c_size
 _OPTIONAL = {}
 
 

@@ -23,6 +23,7 @@ struct BinData {
     WaveCoef w;
     double eta_f;     // eta as the flux sees it (clamped to 1/4 within 1e-4)
     double nu;        // mu / M as RR_force computes it
+    PieceCoef pc;     // H's u-dependence in the common piece form (piece_setup)
 };
 
 constexpr int kNAdd = 80;   // rows of additionalArgs
@@ -50,6 +51,27 @@ constexpr double kE1 = 0.01312004499419488, kE6 = -1.2251564463762044, kE7 = -0.
 constexpr double kBHH1 = 0.2440944881889764, kBHH2 = 0.7338466882816118, kBHH3 = 0.022058823529411766;
 constexpr double kSafe = 0.9, kAlpha = 0.125, kMinScale = 1.0 / 3.0, kMaxScale = 6.0;
 
+// The tableau by source stage for a stage loop that runs (not unrolled):
+// dStage.a[j][i] = A[j + 1 + i][j] (weight of k_j in stage j + 1 + i; 0 past the
+// last stage), dStage.b[j] and dStage.e[j] the weights of k_j in the 8th-order
+// solution and the 5th-order error estimate.
+struct StageTable {
+    double a[12][11];
+    double b[12];
+    double e[12];
+};
+constexpr StageTable make_stage_table() {
+    constexpr double A[12][12] = DOP_A;
+    StageTable t{};
+    for (int j = 0; j < 12; ++j)
+        for (int i = 0; i < 11; ++i) t.a[j][i] = (j + 1 + i < 12) ? A[j + 1 + i][j] : 0.0;
+    for (int j = 0; j < 12; ++j) t.b[j] = t.e[j] = 0.0;
+    t.b[0] = kB1; t.b[5] = kB6; t.b[6] = kB7; t.b[7] = kB8; t.b[8] = kB9; t.b[9] = kB10; t.b[10] = kB11; t.b[11] = kB12;
+    t.e[0] = kE1; t.e[5] = kE6; t.e[6] = kE7; t.e[7] = kE8; t.e[8] = kE9; t.e[9] = kE10; t.e[10] = kE11; t.e[11] = kE12;
+    return t;
+}
+__constant__ static StageTable dStage = make_stage_table();
+
 // ------------------------------------------------------------ group helpers
 template <int G>
 __device__ __forceinline__ double gshfl(double v, int src) {
@@ -61,9 +83,39 @@ __device__ __forceinline__ int gshfl_i(int v, int src) {
     if constexpr (G == 1) return v;
     else return __shfl(v, src, G);
 }
+// lane permutation of a double within rows of 16 lanes (DPP)
+template <int CTRL>
+__device__ __forceinline__ double dpp_perm(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of the wavefront, the same on every lane: xor-1 and
+// xor-2 swaps inside quads, the half-row and row mirrors give every lane its
+// row's sum; the four row sums are then read out and added.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_perm<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_perm<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dpp_perm<0x141>(v);   // row_half_mirror
+    v += dpp_perm<0x140>(v);   // row_mirror
+    const long long b = __double_as_longlong(v);
+    double r[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 16 * q);
+        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 16 * q);
+        r[q] = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+    return (r[0] + r[1]) + (r[2] + r[3]);
+}
+
 template <int G>
 __device__ __forceinline__ double gsum(double v) {
-    if constexpr (G > 1) {
+    if constexpr (G == 64) {
+        return wave_sum(v);
+    } else if constexpr (G > 1) {
 #pragma unroll
         for (int off = G / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, G);
     }
@@ -78,46 +130,54 @@ __device__ __forceinline__ Dual ham_tangent(const HamCoef& hc, double r, double 
     return ham_eval<Dual>(hc, R, PR, LL);
 }
 
-// Everything the flux and the RHS need at one state.
-struct HamPack {
-    double H, dHdr, dHdpr, dHdL, omc;
-};
+// the value of lane `lane` of the wavefront (a scalar register read)
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
 
+// lane q of the group
 template <int G>
-__device__ __forceinline__ HamPack ham_pack(const HamCoef& hc, double r, double pr, double L, int gl) {
-    HamPack p;
-    if constexpr (G >= 4) {
-        const Dual d = ham_tangent(hc, r, pr, L, gl & 3);
-        p.H = gshfl<G>(d.v, 0);
-        p.dHdr = gshfl<G>(d.d, 0);
-        p.dHdpr = gshfl<G>(d.d, 1);
-        p.dHdL = gshfl<G>(d.d, 2);
-        p.omc = gshfl<G>(d.d, 3);
-    } else {
-        double dd[4];
-        double hv = 0.0;
+__device__ __forceinline__ double gread(double v, int q) {
+    if constexpr (G == 64) return read_lane(v, q);
+    else return gshfl<G>(v, q);
+}
+
+// H and its derivatives at one state.  The 13 pieces of H's u-dependence are
+// evaluated on different lanes of the group at once (lane q takes piece q; a
+// group of fewer than 13 lanes takes several per lane), gathered, and
+// combined in closed form on every lane.
+template <int G>
+__device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double pr, double L, int gl) {
+    const double u = 1.0 / r, lr = log(r), su = sqrt(u), hs = 0.5 * su * r;
+    double v[kNumPieces], d[kNumPieces];
+    if constexpr (G >= 16) {
+        const int q = gl < kNumPieces ? gl : 0;
+        double pv, pd;
+        piece_eval(bd.pc.c[q], u, r, lr, su, hs, pv, pd);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            // with G = 2 the lanes split the four tangents and exchange
-            if ((t % G) == (gl % G)) {
-                const Dual d = ham_tangent(hc, r, pr, L, t);
-                dd[t] = d.d;
-                if (t == 0) hv = d.v;
-            } else {
-                dd[t] = 0.0;
-            }
+        for (int k = 0; k < kNumPieces; ++k) {
+            v[k] = gread<G>(pv, k);
+            d[k] = gread<G>(pd, k);
         }
-        if constexpr (G == 1) {
-            p.H = hv; p.dHdr = dd[0]; p.dHdpr = dd[1]; p.dHdL = dd[2]; p.omc = dd[3];
-        } else {
-            p.H = gshfl<G>(hv, 0);
-            p.dHdr = gshfl<G>(dd[0], 0);
-            p.dHdpr = gshfl<G>(dd[1], 1);
-            p.dHdL = gshfl<G>(dd[2], 0);
-            p.omc = gshfl<G>(dd[3], 1);
+    } else {
+        constexpr int PER = (kNumPieces + G - 1) / G;
+        double pv[PER], pd[PER];
+#pragma unroll
+        for (int t = 0; t < PER; ++t) {
+            int q = gl + t * G;
+            if (q >= kNumPieces) q = 0;
+            piece_eval(bd.pc.c[q], u, r, lr, su, hs, pv[t], pd[t]);
+        }
+#pragma unroll
+        for (int k = 0; k < kNumPieces; ++k) {
+            v[k] = gshfl<G>(pv[k / G], k % G);
+            d[k] = gshfl<G>(pd[k / G], k % G);
         }
     }
-    return p;
+    return ham_combine(bd.h, r, u, pr, L, v, d);
 }
 
 // Flux F (before the 1/nu and 1/Omega factors of RR_force), src/EOB.cu:2506-2560,
@@ -143,7 +203,7 @@ __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, co
 template <int G>
 __device__ __forceinline__ void group_rhs(const BinData& bd, const ModeCoef* __restrict__ mc, double r, double pr, double L, int gl,
                                           double k[4]) {
-    const HamPack p = ham_pack<G>(bd.h, r, pr, L, gl);
+    const HamPack p = ham_pack<G>(bd, r, pr, L, gl);
     const double Hreal = bd.nu * p.H;
     const double omega = p.dHdL;   // the ODE overwrites omega with dH/dpphi before RR_force
     double flux = group_flux<G>(mc, bd, r, L, Hreal, omega, p.omc, gl);

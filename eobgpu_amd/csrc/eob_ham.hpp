@@ -207,4 +207,122 @@ EOB_HD T ham_eval(const HamCoef& c, T r, T pr, T L) {
     return c.M * dsqrt(1.0 + 2.0 * c.nu * (Heff - 1.0)) / c.nu;
 }
 
+// ------------------------------------------------------------------ pieces
+// The u-dependence of H as 13 independent pieces of one common form,
+//   piece(u) = sum_{k=0}^{6} (a_k + b_k ln r + s_k sqrt(u)) u^k,
+// so that the value and d/du of every piece can be evaluated together on
+// different lanes with the same instructions (piece_eval), after which
+// ham_combine forms H and its derivatives in closed form.  The pieces are the
+// potentials of ham_eval: AK0, AKa, Dbar, the spin factor of A, Dn, the pr^4,
+// pr^6, pr^8 parts of Q, the gyro factors G0..G3 and gs3 u^2 + so4 u^4.
+enum HamPiece { kAK0, kAKa, kDb, kAS, kDn, kQ4, kQ6, kQ8, kG0, kG1, kG2, kG3, kGS, kNumPieces };
+constexpr int kPieceDeg = 7;
+struct PieceCoef {
+    double c[kNumPieces][kPieceDeg][3];   // (a_k, b_k, s_k) per power u^k
+};
+
+EOB_HD void piece_setup(PieceCoef& p, const HamCoef& c) {
+    for (int q = 0; q < kNumPieces; ++q)
+        for (int k = 0; k < kPieceDeg; ++k) p.c[q][k][0] = p.c[q][k][1] = p.c[q][k][2] = 0.0;
+    auto put = [&](int q, int k, double a, double b = 0.0, double s = 0.0) {
+        p.c[q][k][0] = a; p.c[q][k][1] = b; p.c[q][k][2] = s;
+    };
+    put(kAK0, 0, 1.0); put(kAK0, 1, c.a1); put(kAK0, 2, c.a2); put(kAK0, 3, c.a3); put(kAK0, 4, c.a4);
+    put(kAK0, 5, c.a5, c.a5l); put(kAK0, 6, 0.0, c.a6l, c.a132);
+    put(kAKa, 2, c.ka2); put(kAKa, 3, c.ka3); put(kAKa, 4, c.ka4); put(kAKa, 5, c.ka5); put(kAKa, 6, c.ka6);
+    put(kDb, 0, 1.0); put(kDb, 2, c.d2); put(kDb, 3, c.d3); put(kDb, 4, c.d4, c.d4l); put(kDb, 5, c.d5, c.d5l, c.d112);
+    put(kAS, 0, 1.0); put(kAS, 4, c.as4); put(kAS, 5, c.as5); put(kAS, 6, c.sc6);
+    put(kDn, 0, 1.0); put(kDn, 2, c.ap2); put(kDn, 3, 2.0 * c.ap2);
+    put(kQ4, 2, c.q4_2); put(kQ4, 3, c.q4_3); put(kQ4, 4, c.q4_4, c.q4_4l, c.q4_92);
+    put(kQ6, 2, c.q6_2); put(kQ6, 3, c.q6_3, 0.0, c.q6_72);
+    put(kQ8, 1, c.q8_1); put(kQ8, 2, c.q8_2, 0.0, c.q8_52);
+    put(kG0, 0, c.g00); put(kG0, 1, c.g01); put(kG0, 2, c.g02); put(kG0, 3, c.g03); put(kG0, 4, 0.0, c.g0l);
+    put(kG1, 0, c.g10); put(kG1, 1, c.g11); put(kG1, 2, c.g12); put(kG1, 3, c.g13, c.g1l);
+    put(kG2, 0, c.g20); put(kG2, 1, c.g21); put(kG2, 2, c.g22);
+    put(kG3, 0, c.g30); put(kG3, 1, c.g31);
+    put(kGS, 2, c.gs3); put(kGS, 4, c.so4);
+}
+
+// value and d/du of one piece; r = 1/u, lr = ln r, su = sqrt(u), hs = d su/du = su r / 2
+template <class T>
+EOB_HD void piece_eval(const double (*cf)[3], T u, T r, T lr, T su, T hs, T& v, T& dv) {
+    T acc = cf[6][0] + cf[6][1] * lr + cf[6][2] * su;
+    T dacc = -cf[6][1] * r + cf[6][2] * hs;
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+        const T ck = cf[k][0] + cf[k][1] * lr + cf[k][2] * su;
+        const T dck = -cf[k][1] * r + cf[k][2] * hs;
+        dacc = dck + acc + u * dacc;
+        acc = ck + u * acc;
+    }
+    v = acc;
+    dv = dacc;
+}
+
+// Everything the flux and the RHS need at one state.
+template <class T>
+struct HamPackT {
+    T H, dHdr, dHdpr, dHdL, omc;   // omc: dH/dpphi at pr = 0 (circular frequency)
+};
+using HamPack = HamPackT<double>;
+
+// H, dH/dr, dH/dpr, dH/dpphi and dH/dpphi at pr = 0 from the piece values and
+// u-derivatives (v, d indexed by HamPiece); the closed-form derivatives of
+// ham_eval's expression.
+template <class T>
+EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v, const T* d) {
+    (void)r;
+    const T u2 = u * u, u3 = u2 * u, P = pr * pr, L2 = L * L;
+    const T iDn = 1.0 / v[kDn];
+    const T diDn = -d[kDn] * iDn * iDn;
+    const T f1 = 1.0 - c.e1 * u, df1 = -c.e1;
+    const T fe = f1 + c.e2 * u2, dfe = -c.e1 + 2.0 * c.e2 * u;
+    const T Ks = v[kAK0] + v[kAKa], dKs = d[kAK0] + d[kAKa];
+    const T A0 = Ks * fe * v[kAS];
+    const T A = A0 * iDn;
+    const T dA = (dKs * fe * v[kAS] + Ks * dfe * v[kAS] + Ks * fe * d[kAS]) * iDn + A0 * diDn;
+    const T Bsp = u3 * (c.bs3 + c.bs4 * u), dBsp = u2 * (3.0 * c.bs3 + 4.0 * c.bs4 * u);
+    const T Bnp = v[kAK0] * v[kDb] * f1 + Bsp - 1.0 + c.ap2 * u2;
+    const T dBnp = d[kAK0] * v[kDb] * f1 + v[kAK0] * d[kDb] * f1 + v[kAK0] * v[kDb] * df1 + dBsp + 2.0 * c.ap2 * u;
+    const T m = u2 + 2.0 * u3, dm = 2.0 * u + 6.0 * u2;
+    const T Bnpa = -m * iDn, dBnpa = -dm * iDn - m * diDn;
+    const T W = (Bnpa * c.apsq + 1.0) * u2;   // coefficient of pphi^2
+    const T dW = dBnpa * c.apsq * u2 + (Bnpa * c.apsq + 1.0) * 2.0 * u;
+    const T P2 = P * P;
+    const T Q = P2 * (v[kQ4] + P * (v[kQ6] + P * v[kQ8]));
+    const T dQ = P2 * (d[kQ4] + P * (d[kQ6] + P * d[kQ8]));
+    const T QP = P * (2.0 * v[kQ4] + P * (3.0 * v[kQ6] + 4.0 * P * v[kQ8]));   // dQ/dP
+    const T X = (Bnp + 1.0) * P + W * L2 + Q + 1.0;
+    const T Heven = sqrt(A * X);
+    const T ihe = 0.5 / Heven;
+    const T Gt = v[kG0] + P * (v[kG1] + P * (v[kG2] + P * (v[kG3] + P * c.g40))) + v[kGS];
+    const T dGt = d[kG0] + P * (d[kG1] + P * (d[kG2] + P * d[kG3])) + d[kGS];
+    const T GtP = v[kG1] + P * (2.0 * v[kG2] + P * (3.0 * v[kG3] + 4.0 * P * c.g40));
+    const T U3i = u3 * iDn, dU3i = 3.0 * u2 * iDn + u3 * diDn;
+    const T Heff = Heven + L * U3i * Gt;
+    const T s = sqrt(1.0 + 2.0 * c.nu * (Heff - 1.0));
+    const T kH = c.M / s;   // dH/dHeff
+    const T dHe_du = (dA * X + A * (dBnp * P + dW * L2 + dQ)) * ihe + L * (dU3i * Gt + U3i * dGt);
+    const T dHe_dP = A * (Bnp + 1.0 + QP) * ihe + L * U3i * GtP;
+    const T dHe_dL = A * L * W / Heven + U3i * Gt;
+    const T He0 = sqrt(A * (W * L2 + 1.0));
+    const T Gt0 = v[kG0] + v[kGS];
+    const T s0 = sqrt(1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0));
+    HamPackT<T> p;
+    p.H = c.M * s / c.nu;
+    p.dHdr = -u2 * kH * dHe_du;
+    p.dHdpr = kH * dHe_dP * 2.0 * pr;
+    p.dHdL = kH * dHe_dL;
+    p.omc = (c.M / s0) * (A * L * W / He0 + U3i * Gt0);
+    return p;
+}
+
+// serial form (one lane evaluates all pieces)
+EOB_HD HamPack ham_pack_serial(const HamCoef& c, const PieceCoef& pc, double r, double pr, double L) {
+    const double u = 1.0 / r, lr = log(r), su = sqrt(u), hs = 0.5 * su * r;
+    double v[kNumPieces], d[kNumPieces];
+    for (int q = 0; q < kNumPieces; ++q) piece_eval(pc.c[q], u, r, lr, su, hs, v[q], d[q]);
+    return ham_combine(c, r, u, pr, L, v, d);
+}
+
 }  // namespace eob

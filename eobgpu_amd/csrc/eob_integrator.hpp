@@ -78,7 +78,7 @@ __device__ __forceinline__ void gather_state(const GState<G>& s, int gl, double 
         yf[0] = s.y[0]; yf[1] = s.y[1]; yf[2] = s.y[2]; yf[3] = s.y[3];
     } else if constexpr (G >= 4) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) yf[c] = gshfl<G>(s.y[0], c);
+        for (int c = 0; c < 4; ++c) yf[c] = gread<G>(s.y[0], c);
     } else {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -207,32 +207,37 @@ template <int G>
 __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s, const BinData& bd, const ModeCoef* __restrict__ mc,
                                              const double k1f[4], int gl, bool fixed) {
     constexpr int NC = GState<G>::NC;
-    constexpr double A[12][12] = DOP_A;
-    double k[12][NC];
+    // Stages run in a loop that is not unrolled, so the right-hand side is
+    // emitted once (a compact body that stays in the instruction cache).  Each
+    // stage's k is folded into running sums as soon as it is known:
+    // acc[i] = sum over the stages so far of A[next + i][.] k, plus the 8th-order
+    // (bsum) and error (esum) combinations; the additions happen in the same
+    // order as sum_j A[st][j] k_j, so only the bookkeeping changes.
+    double acc[11][NC], bsum[NC], esum[NC], k0[NC], k8[NC], k11[NC];
+    const double h = s.h;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         const int c = comp_of<G>(gl, q);
-        k[0][q] = k1f[c < 4 ? c : 0];
-    }
-    const double h = s.h;
+        const double kq = k1f[c < 4 ? c : 0];
+        k0[q] = kq;
+        k8[q] = 0.0;
+        k11[q] = 0.0;
 #pragma unroll
+        for (int i = 0; i < 11; ++i) acc[i][q] = dStage.a[0][i] * kq;
+        bsum[q] = kB1 * kq;
+        esum[q] = kE1 * kq;
+    }
+#pragma unroll 1
     for (int st = 1; st < 12; ++st) {
         double arg[NC];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) {
-            double ak = 0.0;
-#pragma unroll
-            for (int j = 0; j < st; ++j) {
-                if (A[st][j] != 0.0) ak += A[st][j] * k[j][q];
-            }
-            arg[q] = s.y[q] + h * ak;
-        }
+        for (int q = 0; q < NC; ++q) arg[q] = s.y[q] + h * acc[0][q];
         double af[4];
         if constexpr (G == 1) {
             af[0] = arg[0]; af[1] = arg[1]; af[2] = arg[2]; af[3] = arg[3];
         } else if constexpr (G >= 4) {
 #pragma unroll
-            for (int c = 0; c < 4; ++c) af[c] = gshfl<G>(arg[0], c);
+            for (int c = 0; c < 4; ++c) af[c] = gread<G>(arg[0], c);
         } else {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
@@ -244,10 +249,18 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
         }
         double kk[4];
         group_rhs<G>(bd, mc, af[0], af[2], af[3], gl, kk);
+        const double bj = dStage.b[st], ej = dStage.e[st];
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
             const int c = comp_of<G>(gl, q);
-            k[st][q] = kk[c < 4 ? c : 0];
+            const double kj = kk[c < 4 ? c : 0];
+#pragma unroll
+            for (int i = 0; i < 10; ++i) acc[i][q] = acc[i + 1][q] + dStage.a[st][i] * kj;
+            acc[10][q] = 0.0;
+            bsum[q] = bsum[q] + bj * kj;
+            esum[q] = esum[q] + ej * kj;
+            if (st == 8) k8[q] = kj;
+            if (st == 11) k11[q] = kj;
         }
     }
     // error norm (src/DOPR853.cu:586-616); stage 11 lives in k2, stage 12 in k3
@@ -255,16 +268,14 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         const int c = comp_of<G>(gl, q);
-        const double temp = kB1 * k[0][q] + kB6 * k[5][q] + kB7 * k[6][q] + kB8 * k[7][q] + kB9 * k[8][q] + kB10 * k[9][q]
-                            + kB11 * k[10][q] + kB12 * k[11][q];
+        const double temp = bsum[q];
         const double yo = s.y[q];
         const double yn = yo + h * temp;
         ynew[q] = yn;
         const double mx = (yo > yn) ? yo : yn;
         const double sk = 1.0 / (P.atol + P.rtol * mx);
-        const double t3 = (temp - kBHH1 * k[0][q] - kBHH2 * k[8][q] - kBHH3 * k[11][q]) * sk;
-        const double t5 = (kE1 * k[0][q] + kE6 * k[5][q] + kE7 * k[6][q] + kE8 * k[7][q] + kE9 * k[8][q] + kE10 * k[9][q]
-                           + kE11 * k[10][q] + kE12 * k[11][q]) * sk;
+        const double t3 = (temp - kBHH1 * k0[q] - kBHH2 * k8[q] - kBHH3 * k11[q]) * sk;
+        const double t5 = esum[q] * sk;
         if (c < 4) {
             const double t3s = t3 * t3, t5s = t5 * t5;
             e3 += t3s * t3s;
@@ -323,12 +334,15 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
 
 // One driver iteration for a group.  Returns false when the group needs a new
 // binary (persistent mode refills; plain mode marks it finished).
-template <int G>
-__device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int gl, bool& need_new) {
+// The tables come from HBM (P.bd, P.mc) or, when the caller staged them in LDS
+// for its binary (lds_bd / lds_mc non-null), from there.
+template <int G, bool kLds = false>
+__device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int gl, bool& need_new,
+                                        const BinData* lds_bd = nullptr, const ModeCoef* lds_mc = nullptr) {
     need_new = false;
     if (s.pass != 1 && s.pass != 2) return;
-    const BinData& bd = P.bd[s.b];
-    const ModeCoef* mc = P.mc + (size_t)s.b * kNumModes;
+    const BinData& bd = kLds ? *lds_bd : P.bd[s.b];
+    const ModeCoef* mc = kLds ? lds_mc : P.mc + (size_t)s.b * kNumModes;
     double yf[4];
     gather_state<G>(s, gl, yf);
     double k1[4];

@@ -314,6 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_setup(const double* add, int n, 
             mc[q].flags = fl;
         }
         bd[i] = d;
+        piece_setup(bd[i].pc, d.h);
     }
 }
 
@@ -362,8 +363,31 @@ __global__ __launch_bounds__(64) void k_initial_conditions(const double* r0, con
     }
 }
 
+// Copy one binary's tables (BinData and its 35 ModeCoef) into this wavefront's
+// LDS slot; the whole wavefront takes part, and the wavefront fence makes the
+// copy visible to its own later reads.
+__device__ __forceinline__ void stage_tables(const IntegParams& P, int b, BinData* lbd, ModeCoef* lmc) {
+    static_assert(sizeof(BinData) % 8 == 0 && sizeof(ModeCoef) % 8 == 0, "tables are copied as 8-byte words");
+    constexpr int nb = sizeof(BinData) / 8, nm = kNumModes * sizeof(ModeCoef) / 8;
+    const int lane = threadIdx.x & 63;
+    const double* gb = reinterpret_cast<const double*>(P.bd + b);
+    const double* gm = reinterpret_cast<const double*>(P.mc + (size_t)b * kNumModes);
+    double* db = reinterpret_cast<double*>(lbd);
+    double* dm = reinterpret_cast<double*>(lmc);
+    for (int i = lane; i < nb; i += 64) db[i] = gb[i];
+    for (int i = lane; i < nm; i += 64) dm[i] = gm[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) {
+    // 64-lane groups own a wavefront: each keeps its binary's tables in LDS
+    constexpr bool kStage = (G == 64);
+    __shared__ BinData lds_bd[kStage ? kBlock / 64 : 1];
+    __shared__ ModeCoef lds_mc[kStage ? kBlock / 64 : 1][kNumModes];
+    const int wave = threadIdx.x >> 6;
     const int gl = threadIdx.x % G;
     GState<G> s;
     s.pass = 0;
@@ -375,12 +399,16 @@ __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) 
             if (gl == 0) b = atomicAdd(P.queue, 1);
             b = gshfl_i<G>(b, 0);
             if (b >= P.nbin) exhausted = true;
-            else begin_binary<G>(P, s, b, gl);
+            else {
+                if constexpr (kStage) stage_tables(P, b, &lds_bd[wave], lds_mc[wave]);
+                begin_binary<G>(P, s, b, gl);
+            }
         }
         if (__all(exhausted)) break;
         if (!exhausted) {
             bool need_new;
-            iterate<G>(P, s, gl, need_new);
+            if constexpr (kStage) iterate<G, true>(P, s, gl, need_new, &lds_bd[wave], lds_mc[wave]);
+            else iterate<G>(P, s, gl, need_new);
         }
     }
 }

@@ -32,6 +32,13 @@ inline int hip_status(hipError_t e, const char* what) {
 
 inline unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Row-batched real FFTs through cached hipFFT plans (spline_kernels.hip).
+// inverse = 0: n rows of N doubles -> n rows of N/2+1 complex (D2Z, unnormalised);
+// inverse = 1: n rows of N/2+1 complex -> n rows of N doubles (Z2D, unnormalised).
+constexpr int kFftChunk = 64;
+int fft_exec(int inverse, const void* in, void* out, int rows, int N, void* stream);
+int fft_rows(int inverse, const void* in, void* out, int n, int N, void* stream);
+
 }  // namespace eobhip_detail
 
 #define EOBHIP_REQUIRE(cond, ...)                        \

@@ -297,21 +297,28 @@ __global__ __launch_bounds__(kBlock) void k_all_in_one_likelihood(double2* temp_
     }
 }
 
-// hipFFT plans, cached per (N, rows per call, stream).  A plan owns a work area,
-// so two streams never share one: work queued on different streams may overlap.
-// Rows go through in chunks of kFftChunk (plus single-row calls for the rest),
-// which bounds the plans per stream to two whatever the batch sizes are.
-constexpr int kFftChunk = 64;
-std::mutex g_plan_mu;
-std::map<std::tuple<long long, int, void*>, hipfftHandle> g_plans;
+}  // namespace
 
-int fft_rows(const double* in, double* out, int rows, int N, void* stream) {
+namespace eobhip_detail {
+
+// hipFFT plans, cached per (N, rows per call, stream, direction).  A plan owns a
+// work area, so two streams never share one: work queued on different streams
+// may overlap.  Callers pass rows in chunks of kFftChunk (plus single rows for
+// the rest), which bounds the plans per stream and size to two.
+namespace {
+std::mutex g_plan_mu;
+std::map<std::tuple<long long, int, void*, int>, hipfftHandle> g_plans;
+}  // namespace
+
+int fft_exec(int inverse, const void* in, void* out, int rows, int N, void* stream) {
+    std::lock_guard<std::mutex> g(g_plan_mu);
     hipfftHandle plan;
-    const auto key = std::make_tuple((long long)N, rows, stream);
+    const auto key = std::make_tuple((long long)N, rows, stream, inverse);
     auto itp = g_plans.find(key);
     if (itp == g_plans.end()) {
         int dims[1] = {N};
-        hipfftResult r = hipfftPlanMany(&plan, 1, dims, nullptr, 1, N, nullptr, 1, N / 2 + 1, HIPFFT_D2Z, rows);
+        hipfftResult r = inverse ? hipfftPlanMany(&plan, 1, dims, nullptr, 1, N / 2 + 1, nullptr, 1, N, HIPFFT_Z2D, rows)
+                                 : hipfftPlanMany(&plan, 1, dims, nullptr, 1, N, nullptr, 1, N / 2 + 1, HIPFFT_D2Z, rows);
         if (r != HIPFFT_SUCCESS) {
             set_error("hipfftPlanMany failed (%d)", (int)r);
             return EOBHIP_EHIP;
@@ -325,15 +332,36 @@ int fft_rows(const double* in, double* out, int rows, int N, void* stream) {
     } else {
         plan = itp->second;
     }
-    hipfftResult r = hipfftExecD2Z(plan, const_cast<double*>(in), reinterpret_cast<hipfftDoubleComplex*>(out));
+    hipfftResult r = inverse ? hipfftExecZ2D(plan, reinterpret_cast<hipfftDoubleComplex*>(const_cast<void*>(in)),
+                                             static_cast<double*>(out))
+                             : hipfftExecD2Z(plan, static_cast<double*>(const_cast<void*>(in)),
+                                             reinterpret_cast<hipfftDoubleComplex*>(out));
     if (r != HIPFFT_SUCCESS) {
-        set_error("hipfftExecD2Z failed (%d)", (int)r);
+        set_error("hipfftExec failed (%d)", (int)r);
         return EOBHIP_EHIP;
     }
     return EOBHIP_OK;
 }
 
-}  // namespace
+int fft_rows(int inverse, const void* in, void* out, int n, int N, void* stream) {
+    const size_t F = (size_t)N / 2 + 1;
+    const size_t in_stride = inverse ? 2 * F : (size_t)N;     // doubles per input row
+    const size_t out_stride = inverse ? (size_t)N : 2 * F;    // doubles per output row
+    const double* src = static_cast<const double*>(in);
+    double* dst = static_cast<double*>(out);
+    int row = 0;
+    for (; row + kFftChunk <= n; row += kFftChunk) {
+        const int rc = fft_exec(inverse, src + row * in_stride, dst + row * out_stride, kFftChunk, N, stream);
+        if (rc) return rc;
+    }
+    for (; row < n; ++row) {
+        const int rc = fft_exec(inverse, src + row * in_stride, dst + row * out_stride, 1, N, stream);
+        if (rc) return rc;
+    }
+    return EOBHIP_OK;
+}
+
+}  // namespace eobhip_detail
 
 extern "C" {
 
@@ -385,18 +413,7 @@ int eobhip_resample_hplus(const double* dense, const double* coef, const int* nu
 // as the reference's rfft, eob/seobnrv4phm.py:742-744).
 int eobhip_rfft_batched(const double* in, double* out, int n, int N, void* stream) {
     EOBHIP_REQUIRE(n >= 1 && N >= 2 && (N % 2) == 0, "bad sizes (n >= 1, N even)");
-    std::lock_guard<std::mutex> g(g_plan_mu);
-    const size_t F = (size_t)N / 2 + 1;
-    int row = 0;
-    for (; row + kFftChunk <= n; row += kFftChunk) {
-        const int rc = fft_rows(in + (size_t)row * N, out + 2 * (size_t)row * F, kFftChunk, N, stream);
-        if (rc) return rc;
-    }
-    for (; row < n; ++row) {
-        const int rc = fft_rows(in + (size_t)row * N, out + 2 * (size_t)row * F, 1, N, stream);
-        if (rc) return rc;
-    }
-    return EOBHIP_OK;
+    return fft_rows(0, in, out, n, N, stream);
 }
 
 int eobhip_fd_likelihood(const double* htilde, const double* dtilde, double* partial, int n, int F, double df, double f0,

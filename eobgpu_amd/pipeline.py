@@ -202,6 +202,85 @@ def fd_likelihood(htilde, dtilde, df, f0=200.0, partial=None, out=None):
     return out
 
 
+class SupportLikelihood:
+    """log L_b = -1/2 4 df sum_k |d~_k - rfft(h_b)_k|^2 / S(f_k) for many templates,
+    computed from each template's support instead of its N-point transform.
+
+    With DD = sum |d~|^2 w, g_j = Re sum_k d~_k w_k e^{2 pi i jk/N} and the lag
+    kernel of w (all fixed by the data set and set up once), a template of L
+    non-zero samples needs h on j < M (M = power of two >= 2L, at most N), one
+    M-point real FFT and two dot products:  log L = -2 df (DD - 2 h.g + hh).
+    The result equals the full-length path (``resample_hplus`` + ``rfft_batched``
+    + ``fd_likelihood``) up to rounding; only the order of the sums differs.
+    """
+
+    def __init__(self, dtilde, N, df, f0=200.0, m_min_log2=10, device="cuda"):
+        L = lib()
+        self.N, self.df, self.f0, self.m_min = int(N), float(df), float(f0), int(m_min_log2)
+        self.log2N = self.N.bit_length() - 1
+        if (1 << self.log2N) != self.N:
+            raise ValueError("N must be a power of two")
+        self.device = device
+        d = torch.as_tensor(dtilde, dtype=torch.complex128, device=device).contiguous()
+        if d.numel() != self.N // 2 + 1:
+            raise ValueError("dtilde must have N/2 + 1 bins")
+        self.g = torch.empty(self.N, dtype=torch.float64, device=device)
+        self.dd = torch.empty(1, dtype=torch.float64, device=device)
+        self.bank = torch.empty(int(L.eobhip_lik_bank_doubles(self.N, self.m_min)), dtype=torch.float64, device=device)
+        scratch = torch.empty(int(L.eobhip_lik_setup_scratch_bytes(self.N)), dtype=torch.uint8, device=device)
+        call("eobhip_lik_setup", d.data_ptr(), self.N, self.df, self.f0, self.m_min, self.g.data_ptr(), self.dd.data_ptr(),
+             self.bank.data_ptr(), scratch.data_ptr(), current_stream())
+        self._keep = (d, scratch)
+
+    def __call__(self, batch, total_mass_msun, dt, amp=None, out=None, terms=None):
+        n, dev = batch.n, self.device
+        st = current_stream()
+        M = torch.as_tensor(np.broadcast_to(np.asarray(total_mass_msun, dtype=np.float64), (n,)).copy(), device=dev)
+        dT_M = (dt / (M * MTSUN_SI)).contiguous()
+        if amp is None:
+            amp_t = torch.ones(n, dtype=torch.float64, device=dev)
+        else:
+            amp_t = torch.as_tensor(np.broadcast_to(np.asarray(amp, dtype=np.float64), (n,)).copy(), device=dev)
+        nsamp = torch.empty(n, dtype=torch.int32, device=dev)
+        mlog2 = torch.empty(n, dtype=torch.int32, device=dev)
+        call("eobhip_lik_sizes", batch.dense.data_ptr(), batch.num_steps.data_ptr(), dT_M.data_ptr(), n, batch.max_step, self.N,
+             self.m_min, nsamp.data_ptr(), mlog2.data_ptr(), st)
+        ml = mlog2.cpu().numpy()
+        order = np.argsort(ml, kind="stable")
+        rowoff = np.empty(n, dtype=np.int64)
+        xoff = np.empty(n, dtype=np.int64)
+        buckets = []
+        ph = px = 0
+        for k in np.unique(ml):
+            idx = order[ml[order] == k]
+            Mk = 1 << int(k)
+            buckets.append((Mk, ph, px, idx.size))
+            rowoff[idx] = ph + Mk * np.arange(idx.size)
+            xoff[idx] = px + (Mk // 2 + 1) * np.arange(idx.size)
+            ph += Mk * idx.size
+            px += (Mk // 2 + 1) * idx.size
+        kmax = int(ml.max()) if n else self.m_min
+        offs = torch.from_numpy(np.stack([rowoff, xoff])).to(dev)
+        hbuf = torch.empty(max(ph, 1), dtype=torch.float64, device=dev)
+        X = torch.empty(max(px, 1), dtype=torch.complex128, device=dev)
+        gx, gx2 = ctypes.c_int(0), ctypes.c_int(0)
+        call("eobhip_lik_partial_blocks", kmax, ctypes.byref(gx), ctypes.byref(gx2))
+        cross = torch.empty(n * gx.value, dtype=torch.float64, device=dev)
+        hh = torch.empty(n * gx2.value, dtype=torch.float64, device=dev)
+        call("eobhip_lik_resample", batch.dense.data_ptr(), batch.coef.data_ptr(), batch.num_steps.data_ptr(), amp_t.data_ptr(),
+             dT_M.data_ptr(), nsamp.data_ptr(), mlog2.data_ptr(), offs[0].data_ptr(), n, batch.max_step, kmax, self.g.data_ptr(),
+             hbuf.data_ptr(), cross.data_ptr(), st)
+        for Mk, h0, x0, cnt in buckets:
+            call("eobhip_rfft_batched", hbuf.data_ptr() + 8 * h0, X.data_ptr() + 16 * x0, int(cnt), int(Mk), st)
+        if out is None:
+            out = torch.empty(n, dtype=torch.float64, device=dev)
+        call("eobhip_lik_finish", X.data_ptr(), offs[1].data_ptr(), mlog2.data_ptr(), n, kmax, self.m_min, self.bank.data_ptr(),
+             cross.data_ptr(), hh.data_ptr(), self.dd.data_ptr(), self.df, out.data_ptr(), None if terms is None else terms.data_ptr(), st)
+        self.nsamp, self.mlog2 = nsamp, mlog2
+        self._tmp = (dT_M, amp_t, offs, hbuf, X, cross, hh)
+        return out
+
+
 def analytic_psd(f, f0=200.0):
     """S(f) = (f/f0)^-4 + 1 + (f/f0)^2 (S(0) = inf)."""
     x = np.asarray(f, dtype=np.float64) / f0

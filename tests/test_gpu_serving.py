@@ -8,7 +8,7 @@ import torch
 
 from eobgpu_amd import configs
 from eobgpu_amd import pipeline as P
-from eobgpu_amd.serving import Request, Server, SlotPool, summarize
+from eobgpu_amd.serving import Request, Server, summarize
 
 pytestmark = pytest.mark.gpu
 
@@ -79,18 +79,3 @@ def test_admission_waits_for_slots(dev):
     assert srv.pool.free == [(0, 8)]
     with pytest.raises(ValueError):
         srv.run([_req(9, 9, 400, 0.0)])
-
-
-def test_slot_pool_ranges():
-    pool = SlotPool.__new__(SlotPool)
-    pool.capacity = 10
-    pool.free = [(0, 10)]
-    a = SlotPool.alloc(pool, 4)
-    b = SlotPool.alloc(pool, 4)
-    assert (a, b) == (0, 4)
-    assert SlotPool.alloc(pool, 3) is None
-    SlotPool.release(pool, a, 4)
-    assert SlotPool.alloc(pool, 3) == 0
-    SlotPool.release(pool, 0, 3)
-    SlotPool.release(pool, 4, 4)
-    assert pool.free == [(0, 10)]
