@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--serve-streams", type=int, default=8)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
     ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
                     help="uneven: the headline 1024-binary pipeline; spline: 512 synthetic (2,2) modes with 5k-60k knots "
                          "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator")
@@ -335,6 +337,14 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.init()
+    if args.copy_probe:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")   # 1 GiB
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 8, "copies": 3}))
+        return
     if args.workload == "spline":
         return workload_spline(args)
     if args.workload == "capacity":

@@ -5,17 +5,20 @@ The persistent kernel has three launch knobs:
 * ``group``: lanes per binary (64 = one binary per wavefront, each RHS spread
   over the wavefront; 16 = four binaries per wavefront, ...);
 * ``block``: threads per workgroup (64, 128 or 256);
-* ``grid_groups``: how many lane groups are launched; every group takes
-  binaries from the device queue until it is empty, so fewer groups than
-  binaries means each group integrates several binaries in turn (refill),
-  and more resident groups than the hardware holds leaves the rest waiting
-  for a slot.
+* ``grid_groups``: how many lane groups are launched (the refill
+  granularity); every group takes binaries from the device queue until it is
+  empty, so fewer groups than binaries means each group integrates several
+  binaries in turn, and more resident groups than the hardware holds leaves
+  the rest waiting for a slot.
 
 ``tune`` times every candidate on the caller's batch (or on a subsample of
 it), keeps the fastest and caches the answer per (batch-size bucket, device
-name).  The trajectories do not depend on the launch shape (each binary's
-arithmetic is the same in every shape; the GPU tests check persistent ==
-plain bit for bit for several groups), so any candidate is a valid answer.
+name).  Block size and grid size do not change any result: each binary's
+arithmetic is the same whichever group, block or launch takes it (the GPU
+tests check persistent == plain bit for bit).  The group width does change
+the order in which the 35 flux modes are summed (a 64-lane tree versus
+partial sums per lane), i.e. results at the rounding level, so ``tune``
+keeps the caller's group (64 by default) unless asked to search groups too.
 """
 from __future__ import annotations
 
@@ -28,7 +31,7 @@ import torch
 
 from . import pipeline as P
 
-DEFAULT_GROUPS = (64, 32, 16, 8)
+DEFAULT_GROUPS = (64,)
 DEFAULT_BLOCKS = (256, 128, 64)
 
 _cache: dict = {}
@@ -49,7 +52,7 @@ def candidates(n, groups=DEFAULT_GROUPS, blocks=DEFAULT_BLOCKS, cu=256):
             # one wavefront per SIMD fits (4 per CU); launch all binaries as groups,
             # or just enough groups to fill the resident slots and refill from the queue
             resident = cu * 4 * (64 // g)
-            for gg in sorted({n, min(n, resident)}):
+            for gg in sorted({n, min(n, resident), min(n, max(per_block, min(n, resident) // 2))}):
                 if gg % per_block and gg != n:
                     continue
                 out.append((g, bl, gg))

@@ -171,6 +171,36 @@ int eobhip_fd_likelihood(const double* htilde, const double* dtilde, double* par
                          double* logl, void* stream);
 size_t eobhip_fd_likelihood_partial_doubles(int n, int F);
 
+/* ------------------------------------------------ likelihood from the support
+ * The same log L as eobhip_fd_likelihood on rfft(h+) (the reference's rfft +
+ * all_in_one_likelihood + get_ll finish, src/interpolate.cu:553-628,
+ * eob/seobnrv4phm.py:742-807), computed as -2 df (DD - 2 sum_j h_j g_j + hh)
+ * with the data-only terms DD, g and the per-M weight bank set up once and hh
+ * from one M-point FFT of h on its support (M = power of two >= 2 x support,
+ * at most N).  N is a power of two; M ranges over 2^m_min .. N.
+ *
+ * Set-up for one data set: dtilde complex [N/2+1] -> g [N], dd [1], bank
+ * [eobhip_lik_bank_doubles(N, m_min_log2)]; scratch eobhip_lik_setup_scratch_bytes(N). */
+size_t eobhip_lik_bank_doubles(int N, int m_min_log2);
+size_t eobhip_lik_setup_scratch_bytes(int N);
+int eobhip_lik_setup(const double* dtilde, int N, double df, double f0, int m_min_log2, double* g, double* dd, double* bank,
+                     void* scratch, void* stream);
+/* Per template: support length nsamp[n] (as eobhip_resample_hplus counts it) and log2 M in mlog2[n]. */
+int eobhip_lik_sizes(const double* dense, const int* num_steps, const double* dT_M, int n, int max_step, int N, int m_min_log2,
+                     int* nsamp, int* mlog2, void* stream);
+/* Partial-sum counts per template for the largest log2 M of the batch (host outputs). */
+int eobhip_lik_partial_blocks(int max_mlog2, int* gx_resample, int* gx_hh);
+/* h on j < M_b at hbuf + rowoff[b] (zero for nsamp_b <= j < M_b) and per-block partials of
+ * sum_j h_j g_j in cross_partial [n][gx_resample]. */
+int eobhip_lik_resample(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M,
+                        const int* nsamp, const int* mlog2, const long long* rowoff, int n, int max_step, int max_mlog2,
+                        const double* g, double* hbuf, double* cross_partial, void* stream);
+/* X (complex, per template M_b/2+1 bins at X + xoff[b], from eobhip_rfft_batched on hbuf) ->
+ * logl[n]; hh_partial [n][gx_hh] scratch; terms (optional, [n][2]) = (sum h g, hh). */
+int eobhip_lik_finish(const double* X, const long long* xoff, const int* mlog2, int n, int max_mlog2, int m_min_log2,
+                      const double* bank, const double* cross_partial, double* hh_partial, const double* dd, double df, double* logl,
+                      double* terms, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
