@@ -145,6 +145,22 @@ __device__ __forceinline__ double gread(double v, int q) {
     else return gshfl<G>(v, q);
 }
 
+// two square roots / reciprocals as one call on lanes 0 and 1 of the group
+template <int G>
+struct LanePair {
+    int gl;
+    __device__ void sqrt2(double a, double b, double& x, double& y) const {
+        const double t = sqrt((gl & 1) ? b : a);
+        x = gread<G>(t, 0);
+        y = gread<G>(t, 1);
+    }
+    __device__ void rcp2(double a, double b, double& x, double& y) const {
+        const double t = 1.0 / ((gl & 1) ? b : a);
+        x = gread<G>(t, 0);
+        y = gread<G>(t, 1);
+    }
+};
+
 // H and its derivatives at one state.  The 13 pieces of H's u-dependence are
 // evaluated on different lanes of the group at once (lane q takes piece q; a
 // group of fewer than 13 lanes takes several per lane), gathered, and
@@ -177,7 +193,8 @@ __device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double 
             d[k] = gshfl<G>(pd[k / G], k % G);
         }
     }
-    return ham_combine(bd.h, r, u, pr, L, v, d);
+    if constexpr (G >= 2) return ham_combine(bd.h, r, u, pr, L, v, d, LanePair<G>{gl});
+    else return ham_combine(bd.h, r, u, pr, L, v, d);
 }
 
 // Flux F (before the 1/nu and 1/Omega factors of RR_force), src/EOB.cu:2506-2560,
@@ -185,10 +202,19 @@ __device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double 
 template <int G>
 __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, const BinData& bd, double r, double L, double Hreal,
                                              double omega, double omc, int gl) {
-    const double v = cbrt(omega);
-    const double logv = log(v);
     const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
-    const double vPhi = r * cbrt(vPhiIn) * omega;
+    double v, cb_in;
+    if constexpr (G >= 4) {
+        // the two cube roots in one call, on two lanes
+        const double cr = cbrt((gl & 1) ? vPhiIn : omega);
+        v = gread<G>(cr, 0);
+        cb_in = gread<G>(cr, 1);
+    } else {
+        v = cbrt(omega);
+        cb_in = cbrt(vPhiIn);
+    }
+    const double logv = log(v);
+    const double vPhi = r * cb_in * omega;
     const double vPhi2 = vPhi * vPhi;
     const double Heff = (Hreal * Hreal - 1.0) / (2.0 * bd.eta_f) + 1.0;
     const double E1 = exp(-4.0 * kPi * Hreal * omega);

@@ -266,11 +266,21 @@ struct HamPackT {
 };
 using HamPack = HamPackT<double>;
 
+// Two independent square roots or reciprocals.  On one lane they are two
+// calls; a lane group (eob_device.hpp) evaluates them as one call on two lanes.
+struct SerialPair {
+    template <class T>
+    EOB_HD void sqrt2(T a, T b, T& x, T& y) const { x = sqrt(a); y = sqrt(b); }
+    template <class T>
+    EOB_HD void rcp2(T a, T b, T& x, T& y) const { x = 1.0 / a; y = 1.0 / b; }
+};
+
 // H, dH/dr, dH/dpr, dH/dpphi and dH/dpphi at pr = 0 from the piece values and
 // u-derivatives (v, d indexed by HamPiece); the closed-form derivatives of
-// ham_eval's expression.
-template <class T>
-EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v, const T* d) {
+// ham_eval's expression.  The pr = 0 branch (for the circular frequency) runs
+// beside the full one, its square roots and reciprocals paired with theirs.
+template <class T, class Pair = SerialPair>
+EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v, const T* d, const Pair& pair = Pair()) {
     (void)r;
     const T u2 = u * u, u3 = u2 * u, P = pr * pr, L2 = L * L;
     const T iDn = 1.0 / v[kDn];
@@ -293,27 +303,29 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     const T dQ = P2 * (d[kQ4] + P * (d[kQ6] + P * d[kQ8]));
     const T QP = P * (2.0 * v[kQ4] + P * (3.0 * v[kQ6] + 4.0 * P * v[kQ8]));   // dQ/dP
     const T X = (Bnp + 1.0) * P + W * L2 + Q + 1.0;
-    const T Heven = sqrt(A * X);
-    const T ihe = 0.5 / Heven;
+    T Heven, He0, iHe, iHe0;
+    pair.sqrt2(A * X, A * (W * L2 + 1.0), Heven, He0);
+    pair.rcp2(Heven, He0, iHe, iHe0);
+    const T ihe = 0.5 * iHe;
     const T Gt = v[kG0] + P * (v[kG1] + P * (v[kG2] + P * (v[kG3] + P * c.g40))) + v[kGS];
     const T dGt = d[kG0] + P * (d[kG1] + P * (d[kG2] + P * d[kG3])) + d[kGS];
     const T GtP = v[kG1] + P * (2.0 * v[kG2] + P * (3.0 * v[kG3] + 4.0 * P * c.g40));
     const T U3i = u3 * iDn, dU3i = 3.0 * u2 * iDn + u3 * diDn;
     const T Heff = Heven + L * U3i * Gt;
-    const T s = sqrt(1.0 + 2.0 * c.nu * (Heff - 1.0));
-    const T kH = c.M / s;   // dH/dHeff
+    const T Gt0 = v[kG0] + v[kGS];
+    T s, s0, is, is0;
+    pair.sqrt2(1.0 + 2.0 * c.nu * (Heff - 1.0), 1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0), s, s0);
+    pair.rcp2(s, s0, is, is0);
+    const T kH = c.M * is;   // dH/dHeff
     const T dHe_du = (dA * X + A * (dBnp * P + dW * L2 + dQ)) * ihe + L * (dU3i * Gt + U3i * dGt);
     const T dHe_dP = A * (Bnp + 1.0 + QP) * ihe + L * U3i * GtP;
-    const T dHe_dL = A * L * W / Heven + U3i * Gt;
-    const T He0 = sqrt(A * (W * L2 + 1.0));
-    const T Gt0 = v[kG0] + v[kGS];
-    const T s0 = sqrt(1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0));
+    const T dHe_dL = A * L * W * iHe + U3i * Gt;
     HamPackT<T> p;
     p.H = c.M * s / c.nu;
     p.dHdr = -u2 * kH * dHe_du;
     p.dHdpr = kH * dHe_dP * 2.0 * pr;
     p.dHdL = kH * dHe_dL;
-    p.omc = (c.M / s0) * (A * L * W / He0 + U3i * Gt0);
+    p.omc = (c.M * is0) * (A * L * W * iHe0 + U3i * Gt0);
     return p;
 }
 
