@@ -62,12 +62,14 @@ _SIGS.update({
     "eobhip_interpolate_TD": [c_vp] * 7 + [c_int, c_int, c_vp],
     "eobhip_all_in_one_likelihood": [c_vp] * 8 + [c_double, c_int, c_int, c_int, c_vp],
     "eobhip_spline_build_batched": [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp],
+    "eobhip_spline_scratch_doubles": [c_int, c_int],
     "eobhip_resample_hplus": [c_vp] * 5 + [c_int, c_int, c_int, c_vp, c_vp, c_vp],
     "eobhip_rfft_batched": [c_vp, c_vp, c_int, c_int, c_vp],
     "eobhip_fd_likelihood": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_vp, c_vp],
     "eobhip_fd_likelihood_partial_doubles": [c_int, c_int],
 })
 _RESTYPES["eobhip_fd_likelihood_partial_doubles"] = c_size
+_RESTYPES["eobhip_spline_scratch_doubles"] = c_size
 _SIGS.update({
     "eobhip_lik_bank_doubles": [c_int, c_int],
     "eobhip_lik_setup_scratch_bytes": [c_int],

@@ -106,47 +106,88 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_TD(const double* __restr
 // ---------------------------------------------------- batched EOB splines
 // Three interpolants per binary over its trajectory knots t_k (units of M):
 // q = 0 phi, 1 Re(h22 e^{2 i phi}), 2 Im(h22 e^{2 i phi}).  coef [n][max_step][12]
-// holds (y, c1, c2, c3) per interpolant; scratch [n][3][max_step] holds slopes.
-__global__ __launch_bounds__(64) void k_spline_build(const double* __restrict__ dense, const double* __restrict__ knots,
-                                                     const int* __restrict__ num_steps, int n, int max_step, double* coef,
-                                                     double* scratch) {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= 3 * n) return;
-    const int b = tid / 3, q = tid % 3;
+// holds (y, c1, c2, c3) per interpolant.
+// The same splines with one workgroup per interpolant.  The slope system is
+// split into up to 256 chunks of >= kChunkMin rows; each thread solves its
+// chunk together with kHalo rows on either side as a system of its own (cut
+// off where the halo ends) and keeps the slopes of its own rows.  Inside the
+// system every row is diagonally dominant (2(dx1 + dx2) against dx1 + dx2),
+// so the effect of the cut on a row k rows away is damped by at least 2^-k:
+// with 64 halo rows it is below 1e-19 of the neighbouring slopes, i.e. the
+// chunked solve returns the slopes of the whole system to rounding.  The
+// not-a-knot end rows are never cut (the first and last chunks start and end
+// there).  The coefficients are then formed by all threads in parallel.
+constexpr int kHalo = 64;
+constexpr int kChunkMin = 128;
+constexpr int kSplineThreads = 256;
+
+__host__ __device__ inline int spline_chunks(int len) {
+    int P = len / kChunkMin;
+    return P < 1 ? 1 : (P > kSplineThreads ? kSplineThreads : P);
+}
+// the largest chunk any len <= max_step gives: len itself below 2 kChunkMin,
+// under 1.5 kChunkMin up to 256 chunks, then ceil(len / 256)
+__host__ __device__ inline int spline_mmax(int max_step) {
+    const int small = max_step < 2 * kChunkMin ? max_step : 2 * kChunkMin;
+    const int big = (max_step + kSplineThreads - 1) / kSplineThreads;
+    return small > big ? small : big;
+}
+// per-interpolant scratch in doubles: slopes + per-chunk (c', d') rows
+__host__ __device__ inline long long spline_scratch_per_interp(int max_step) {
+    return (long long)max_step + (long long)kSplineThreads * 2 * (spline_mmax(max_step) + 2 * kHalo);
+}
+
+__global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const double* __restrict__ dense, const double* __restrict__ knots,
+                                                                     const int* __restrict__ num_steps, int n, int max_step,
+                                                                     double* coef, double* scratch) {
+    const int b = blockIdx.x / 3, q = blockIdx.x % 3;
+    if (b >= n) return;
     const int len = num_steps[b];
-    const double* T = dense + (size_t)b * max_step * 5;   // t at stride 5
-    const double* Y = knots + (size_t)b * max_step * 3 + q;  // stride 3
-    double* C = coef + (size_t)b * max_step * 12 + 4 * q;    // stride 12
-    double* S = scratch + ((size_t)b * 3 + q) * max_step;
+    const double* T = dense + (size_t)b * max_step * 5;
+    const double* Y = knots + (size_t)b * max_step * 3 + q;
+    double* C = coef + (size_t)b * max_step * 12 + 4 * q;
+    double* S = scratch + ((size_t)b * 3 + q) * spline_scratch_per_interp(max_step);
     if (len < 3) {
-        for (int i = 0; i < len; ++i) { C[12 * i] = Y[3 * i]; C[12 * i + 1] = C[12 * i + 2] = C[12 * i + 3] = 0.0; }
+        for (int i = threadIdx.x; i < len; i += blockDim.x) {
+            C[12 * i] = Y[3 * i];
+            C[12 * i + 1] = C[12 * i + 2] = C[12 * i + 3] = 0.0;
+        }
         return;
     }
-    // forward elimination; keep modified diag in C[.+2], ud in C[.+1] temporarily
-    double pdg = 0.0, pud = 0.0, pb = 0.0;
-    for (int i = 0; i < len; ++i) {
-        int i0, i1, i2;
-        if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
-        else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
-        else { i0 = i - 1; i1 = i; i2 = i + 1; }
-        Row r = nak_row(i, len, T[5 * i0], T[5 * i1], T[5 * i2], Y[3 * i0], Y[3 * i1], Y[3 * i2]);
-        if (i > 0) {
-            const double w = r.ld / pdg;
-            r.dg = r.dg - w * pud;
-            r.b = r.b - w * pb;
+    const int P = spline_chunks(len);
+    const int m = (len + P - 1) / P;
+    const int mmax = spline_mmax(max_step);
+    const int p = threadIdx.x;
+    if (p < P) {
+        const int s0 = p * m, e0 = (s0 + m < len) ? s0 + m : len;
+        if (s0 < e0) {
+            const int lo = (s0 - kHalo > 0) ? s0 - kHalo : 0;
+            const int hi = (e0 + kHalo < len) ? e0 + kHalo : len;
+            double* cw = S + max_step + (size_t)p * 2 * (mmax + 2 * kHalo);   // c' and d' of rows lo..hi-1
+            double pc = 0.0, pd = 0.0;
+            for (int i = lo; i < hi; ++i) {
+                int i0, i1, i2;
+                if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
+                else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
+                else { i0 = i - 1; i1 = i; i2 = i + 1; }
+                const Row r = nak_row(i, len, T[5 * i0], T[5 * i1], T[5 * i2], Y[3 * i0], Y[3 * i1], Y[3 * i2]);
+                const double a = (i == lo) ? 0.0 : r.ld;          // cut below the halo
+                const double c = (i == hi - 1) ? 0.0 : r.ud;      // cut above the halo
+                const double den = r.dg - a * pc;
+                pc = c / den;
+                pd = (r.b - a * pd) / den;
+                cw[2 * (i - lo)] = pc;
+                cw[2 * (i - lo) + 1] = pd;
+            }
+            double x = 0.0;
+            for (int i = hi - 1; i >= lo; --i) {
+                x = cw[2 * (i - lo) + 1] - cw[2 * (i - lo)] * x;
+                if (i >= s0 && i < e0) S[i] = x;
+            }
         }
-        C[12 * i + 1] = r.ud;
-        C[12 * i + 2] = r.dg;
-        S[i] = r.b;
-        pdg = r.dg; pud = r.ud; pb = r.b;
     }
-    double sn = S[len - 1] / C[12 * (len - 1) + 2];
-    S[len - 1] = sn;
-    for (int i = len - 2; i >= 0; --i) {
-        sn = (S[i] - C[12 * i + 1] * sn) / C[12 * i + 2];
-        S[i] = sn;
-    }
-    for (int i = 0; i < len - 1; ++i) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < len - 1; i += blockDim.x) {
         const double dt = T[5 * (i + 1)] - T[5 * i];
         const double yi = Y[3 * i], yp = Y[3 * (i + 1)];
         const double slope = (yp - yi) / dt;
@@ -157,8 +198,10 @@ __global__ __launch_bounds__(64) void k_spline_build(const double* __restrict__ 
         C[12 * i + 2] = (slope - si) / dt - tt;
         C[12 * i + 3] = tt / dt;
     }
-    C[12 * (len - 1)] = Y[3 * (len - 1)];
-    C[12 * (len - 1) + 1] = C[12 * (len - 1) + 2] = C[12 * (len - 1) + 3] = 0.0;
+    if (threadIdx.x == 0) {
+        C[12 * (len - 1)] = Y[3 * (len - 1)];
+        C[12 * (len - 1) + 1] = C[12 * (len - 1) + 2] = C[12 * (len - 1) + 3] = 0.0;
+    }
 }
 
 // Uniform resample of h+ = amp * Re[(Re S + i Im S)(t) e^{-2 i phi(t)}] at
@@ -389,11 +432,17 @@ int eobhip_all_in_one_likelihood(double* temp_sum, const double* hp_fd, const do
     return check_launch("all_in_one_likelihood");
 }
 
+size_t eobhip_spline_scratch_doubles(int n, int max_step) {
+    if (n < 0 || max_step < 1) return 0;
+    return (size_t)n * 3 * (size_t)spline_scratch_per_interp(max_step);
+}
+
 int eobhip_spline_build_batched(const double* dense, const double* knots, const int* num_steps, int n, int max_step, double* coef,
                                 double* scratch, void* stream) {
     EOBHIP_REQUIRE(n >= 0 && max_step >= 1, "bad sizes");
+    EOBHIP_REQUIRE(3LL * n <= 0x7fffffffLL, "too many interpolants");
     if (n == 0) return EOBHIP_OK;
-    hipLaunchKernelGGL(k_spline_build, dim3(blocks_for(3LL * n, 64)), dim3(64), 0, as_stream(stream), dense, knots, num_steps, n, max_step,
+    hipLaunchKernelGGL(k_spline_build_par, dim3(3 * n), dim3(kSplineThreads), 0, as_stream(stream), dense, knots, num_steps, n, max_step,
                        coef, scratch);
     return check_launch("spline_build");
 }

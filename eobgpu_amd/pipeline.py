@@ -159,7 +159,8 @@ class EOBBatch:
         """Not-a-knot splines of phi, Re and Im of h22 e^{2 i phi} over each binary's knots."""
         if getattr(self, "coef", None) is None:
             self.coef = torch.empty((self.n, self.max_step, 12), dtype=torch.float64, device=self.device)
-            self.spline_scratch = torch.empty((self.n, 3, self.max_step), dtype=torch.float64, device=self.device)
+            self.spline_scratch = torch.empty(int(lib().eobhip_spline_scratch_doubles(self.n, self.max_step)), dtype=torch.float64,
+                                              device=self.device)
         call("eobhip_spline_build_batched", self.dense.data_ptr(), self.knots.data_ptr(), self.num_steps.data_ptr(), self.n,
              self.max_step, self.coef.data_ptr(), self.spline_scratch.data_ptr(), current_stream())
         return self.coef

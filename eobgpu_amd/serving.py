@@ -77,7 +77,8 @@ class SlotPool:
         self.dense = torch.zeros((c, ms, 5), dtype=torch.float64, device=device)
         self.knots = torch.zeros((c, ms, 3), dtype=torch.float64, device=device)
         self.coef = torch.zeros((c, ms, 12), dtype=torch.float64, device=device)
-        self.spl = torch.zeros((c, 3, ms), dtype=torch.float64, device=device)
+        self.spl_per_bin = int(L.eobhip_spline_scratch_doubles(1, ms))
+        self.spl = torch.zeros(c * self.spl_per_bin, dtype=torch.float64, device=device)
         self.hp = torch.zeros((c, self.N), dtype=torch.float64, device=device)
         self.ht = torch.zeros((c, self.F), dtype=torch.complex128, device=device)
         self.num_steps = torch.zeros(c, dtype=torch.int32, device=device)
@@ -90,7 +91,7 @@ class SlotPool:
 
     def bytes_per_binary(self):
         ms = self.max_step
-        return self.bd_bytes + self.mode_bytes + 8 * ms * (5 + 3 + 12 + 3) + 8 * self.N + 16 * self.F + 4 * 5 + 8 * 2
+        return self.bd_bytes + self.mode_bytes + 8 * ms * (5 + 3 + 12) + 8 * self.spl_per_bin + 8 * self.N + 16 * self.F + 4 * 5 + 8 * 2
 
     def alloc(self, nb):
         for i, (s, ln) in enumerate(self.free):
@@ -162,7 +163,7 @@ class Server:
             call("eobhip_hlm22_knots", dense.data_ptr(), pool.num_steps[sl].data_ptr(), bd.data_ptr(), md.data_ptr(), nb,
                  pool.max_step, pool.knots[sl].data_ptr(), s)
             call("eobhip_spline_build_batched", dense.data_ptr(), pool.knots[sl].data_ptr(), pool.num_steps[sl].data_ptr(), nb,
-                 pool.max_step, pool.coef[sl].data_ptr(), pool.spl[sl].data_ptr(), s)
+                 pool.max_step, pool.coef[sl].data_ptr(), pool.spl[start * pool.spl_per_bin:].data_ptr(), s)
             amp = torch.ones(nb, dtype=torch.float64, device=self.device)
             call("eobhip_resample_hplus", dense.data_ptr(), pool.coef[sl].data_ptr(), pool.num_steps[sl].data_ptr(), amp.data_ptr(),
                  dT_M.data_ptr(), nb, pool.max_step, self.N, pool.hp[sl].data_ptr(), pool.nsamp[sl].data_ptr(), s)

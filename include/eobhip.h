@@ -152,8 +152,11 @@ int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bi
                        int max_step, double* knots, void* stream);
 
 /* Not-a-knot splines of the three knot series (phi and Re, Im of h22 e^{2 i phi}) per
- * binary (CubicSplineInterpolantTD / interpolate_TD semantics, src/interpolate.cu:11-308).
- * coef [n][max_step][12] = (y, c1, c2, c3) x 3; scratch [n][3][max_step] doubles. */
+ * binary (CubicSplineInterpolantTD / interpolate_TD semantics, src/interpolate.cu:11-308),
+ * one workgroup per series.  coef [n][max_step][12] = (y, c1, c2, c3) x 3;
+ * scratch: eobhip_spline_scratch_doubles(n, max_step) doubles (binary b's part starts at
+ * b * eobhip_spline_scratch_doubles(1, max_step)). */
+size_t eobhip_spline_scratch_doubles(int n, int max_step);
 int eobhip_spline_build_batched(const double* dense, const double* knots, const int* num_steps, int n, int max_step,
                                 double* coef, double* scratch, void* stream);
 /* h+ = amp * Re[h22(t_j)] on t_j = t_0 + j dT_M (dT_M [n] in units of M), j < N; zero past the
