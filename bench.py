@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
-    ap.add_argument("--serve-streams", type=int, default=8)
+    ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--copy-probe", action="store_true",
@@ -215,7 +215,8 @@ def serve_section(args):
     data = h0 + sig * (rng.normal(size=F) + 1j * rng.normal(size=F)) / np.sqrt(2)
     reqs = [Request(i, t, p) for i, (t, p) in enumerate(configs.served_requests(duration_s=args.serve_seconds))]
     srv = Server(data, df=df, dt=dt, N=N, capacity=8192, max_step=2048, streams=args.serve_streams)
-    srv.run([Request(-1 - i, 0.0, configs.small_batch(n=k, seed=900 + i)) for i, k in enumerate((64, 1, 64, 1))])
+    srv.warm()
+    srv.run([Request(-1, 0.0, configs.small_batch(n=64, seed=900))])
     out = {"requests": len(reqs), "N": N, "dt": dt, "streams": args.serve_streams}
     for policy in ("continuous", "plain"):
         res = srv.run(reqs, policy=policy)

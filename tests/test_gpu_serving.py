@@ -35,7 +35,7 @@ def _server(capacity=64, streams=4):
 def test_alone_and_in_the_mix(dev):
     reqs = [_req(0, 5, 101, 0.0), _req(1, 3, 102, 0.0005), _req(2, 7, 103, 0.001), _req(3, 2, 104, 0.001), _req(4, 9, 105, 0.002)]
     srv = _server()
-    srv.run([_req(9, 4, 99, 0.0)])      # warm the per-stream FFT plans
+    srv.warm()
     alone = {}
     for r in reqs:
         res = srv.run([Request(r.rid, 0.0, r.params)])
@@ -45,6 +45,12 @@ def test_alone_and_in_the_mix(dev):
     mix = srv.run(reqs)
     assert sorted(x.rid for x in mix) == [r.rid for r in reqs]
     for x in mix:
+        np.testing.assert_array_equal(x.logl, alone[x.rid])
+    # all at once: admitted as one launch group
+    burst = srv.run([Request(r.rid, 0.0, r.params) for r in reqs])
+    assert sorted(x.rid for x in burst) == [r.rid for r in reqs]
+    assert len({x.admitted for x in burst}) == 1
+    for x in burst:
         np.testing.assert_array_equal(x.logl, alone[x.rid])
     plain = srv.run(reqs, policy="plain")
     for x in plain:
