@@ -167,7 +167,7 @@ struct LanePair {
 // combined in closed form on every lane.
 template <int G>
 __device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double pr, double L, int gl) {
-    const double u = 1.0 / r, lr = log(r), su = sqrt(u), hs = 0.5 * su * r;
+    const double u = 1.0 / r, lr = log_pos(r), su = sqrt(u), hs = 0.5 * su * r;
     double v[kNumPieces], d[kNumPieces];
     if constexpr (G >= 16) {
         const int q = gl < kNumPieces ? gl : 0;
@@ -213,14 +213,18 @@ __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, co
         v = cbrt(omega);
         cb_in = cbrt(vPhiIn);
     }
-    const double logv = log(v);
+    const double logv = log_pos(v);
     const double vPhi = r * cb_in * omega;
     const double vPhi2 = vPhi * vPhi;
     const double Heff = (Hreal * Hreal - 1.0) / (2.0 * bd.eta_f) + 1.0;
     const double E1 = exp(-4.0 * kPi * Hreal * omega);
     double part = 0.0;
-    for (int i = gl; i < kNumModes; i += G) {
-        part += flux_mode(mc[i], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+    if constexpr (G >= kNumModes) {
+        // one mode per lane; the spare lanes repeat the last mode and drop it
+        const double fm = flux_mode(mc[gl < kNumModes ? gl : kNumModes - 1], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+        part = (gl < kNumModes) ? fm : 0.0;
+    } else {
+        for (int i = gl; i < kNumModes; i += G) part += flux_mode(mc[i], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
     }
     return gsum<G>(part) / kPi / 8.0;
 }

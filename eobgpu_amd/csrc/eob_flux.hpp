@@ -306,7 +306,7 @@ template <class T>
 EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T E1, T Heff, T pphi, T vPhi2) {
     const T hk = H * (double)z.m * Omega;
     const T x = 4.0 * kPi * hk;
-    const T Em = ipow(E1, z.m);
+    const T Em = ipow15(E1, z.m);   // m, l <= 8 and l + eps <= 9
     T prod = 1.0;
     const T hk4 = 4.0 * hk * hk;
 #pragma unroll
@@ -328,10 +328,10 @@ EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T E1, T Heff, T
     acc = acc * v + z.c[0];
     const T rho = 1.0 + acc * v;
     const T aux = v * (z.f1 + v * v * z.f3);
-    T rpl = ipow(rho, z.l);
+    T rpl = ipow15(rho, z.l);
     rpl = (z.flags & 2) ? aux : rpl + aux;
     const T S = (z.flags & 1) ? Heff : v * pphi;
-    const T hN2 = z.N2 * ipow(vPhi2, z.lpe);
+    const T hN2 = z.N2 * ipow15(vPhi2, z.lpe);
     return z.m2 * Omega * Omega * T2 * S * S * rpl * rpl * hN2;
 }
 

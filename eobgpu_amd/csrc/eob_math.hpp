@@ -11,6 +11,7 @@
 #include <cmath>
 
 #define EOB_HD __host__ __device__ __forceinline__
+#include "eob_fastmath.hpp"
 
 namespace eob {
 
@@ -104,6 +105,20 @@ EOB_HD T ipow(T x, int n) {
         x = x * x;
         n >>= 1;
     }
+    return r;
+}
+
+// x^n for 0 <= n < 16: the same products as ipow, as straight-line code with
+// selects (lanes with different n then share one instruction stream)
+template <class T>
+EOB_HD T ipow15(T x, int n) {
+    T r = (n & 1) ? x : T(1.0);
+    x = x * x;
+    r = (n & 2) ? r * x : r;
+    x = x * x;
+    r = (n & 4) ? r * x : r;
+    x = x * x;
+    r = (n & 8) ? r * x : r;
     return r;
 }
 

@@ -1,0 +1,66 @@
+"""The right-hand side's logarithm (eob_fastmath.hpp, log_pos) against
+correctly rounded logarithms from Python's decimal module: within 1 ulp over
+the radii and frequencies the integrator sees and over the whole exponent
+range; NaN outside the positive finite numbers.  The header is compiled for
+the host with contraction off, which rounds as the device build does."""
+import ctypes
+import math
+import os
+import shutil
+import subprocess
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def probe(tmp_path_factory):
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    out = str(tmp_path_factory.mktemp("fastmath") / "probe.so")
+    subprocess.run([cxx, "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", out,
+                    os.path.join(HERE, "fastmath_probe.cpp")], check=True)
+    lib = ctypes.CDLL(out)
+    lib.probe_log_pos.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+
+    def run(x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        lib.probe_log_pos(x.ctypes.data, y.ctypes.data, len(x))
+        return y
+    return run
+
+
+def _ulp_err(x, y):
+    getcontext().prec = 40
+    worst = 0.0
+    for xi, yi in zip(x, y):
+        ref = Decimal(float(xi)).ln()
+        r = float(ref)
+        u = math.ulp(r) if r != 0.0 else math.ulp(0.0)
+        worst = max(worst, float(abs(Decimal(float(yi)) - ref) / Decimal(u)))
+    return worst
+
+
+def test_within_one_ulp(probe):
+    rng = np.random.default_rng(11)
+    x = np.concatenate([
+        rng.uniform(1.0, 100.0, 8000),          # radii
+        10.0 ** rng.uniform(-2.5, 0.0, 8000),   # v = Omega^(1/3)
+        rng.uniform(0.70, 1.42, 4000),          # around the reduction's split points
+        2.0 ** rng.uniform(-1070.0, 1023.0, 4000),
+        np.array([1.0, 2.0, 0.5, math.sqrt(0.5), np.nextafter(math.sqrt(0.5), 1.0), np.nextafter(1.0, 0.0),
+                  np.nextafter(1.0, 2.0), 5e-324, 1.7976931348623157e308]),
+    ])
+    y = probe(x)
+    assert y[-9] == 0.0 and y[-8] == math.log(2.0)
+    assert _ulp_err(x, y) <= 1.0
+
+
+def test_non_positive_and_non_finite(probe):
+    y = probe(np.array([0.0, -0.0, -1.0, -np.inf, np.inf, np.nan]))
+    assert np.all(np.isnan(y))

@@ -46,10 +46,15 @@ def test_alone_and_in_the_mix(dev):
     assert sorted(x.rid for x in mix) == [r.rid for r in reqs]
     for x in mix:
         np.testing.assert_array_equal(x.logl, alone[x.rid])
-    # all at once: admitted as one launch group
-    burst = srv.run([Request(r.rid, 0.0, r.params) for r in reqs])
+    # all at once, with a group limit that holds them: admitted as one launch group
+    burst = srv.run([Request(r.rid, 0.0, r.params) for r in reqs], max_group=64)
     assert sorted(x.rid for x in burst) == [r.rid for r in reqs]
     assert len({x.admitted for x in burst}) == 1
+    # the default limit (capacity / streams = 16 binaries) splits the same burst
+    split = srv.run([Request(r.rid, 0.0, r.params) for r in reqs])
+    assert len({x.admitted for x in split}) > 1
+    for x in split:
+        np.testing.assert_array_equal(x.logl, alone[x.rid])
     for x in burst:
         np.testing.assert_array_equal(x.logl, alone[x.rid])
     plain = srv.run(reqs, policy="plain")
