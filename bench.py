@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 # FP64 operations per attempted DOP853 step (12 right-hand sides + stages +
 # error norm + controller), counted by eobgpu_amd/csrc/flop_count.cpp on the
 # library's own Hamiltonian and flux code.
-STEP_FLOPS = 50147
+STEP_FLOPS = 49691
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 

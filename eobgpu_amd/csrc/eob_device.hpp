@@ -5,11 +5,11 @@
 //
 // Lane groups.  A binary is integrated by a group of G consecutive lanes of one
 // wavefront (G = 1, 4, 8, 16, 32 or 64).  Inside the right-hand side
-//   - lanes 0..3 each evaluate H with one forward-mode tangent:
-//       lane 0 d/dr, lane 1 d/dpr, lane 2 d/dpphi, lane 3 d/dpphi at pr = 0
-//     (the reference evaluates two full gradients and a third H per RHS,
-//      src/EOB.cu:3102, :2612, :2619; the values are identical);
-//   - the 35 flux modes are spread over the G lanes and reduced with shuffles;
+//   - the 13 pieces of H's u-dependence are evaluated on different lanes and
+//     gathered; H, dH/dr, dH/dpr, dH/dpphi and dH/dpphi at pr = 0 follow in
+//     closed form on every lane (the reference evaluates two full gradients
+//     and a third H per RHS, src/EOB.cu:3102, :2612, :2619);
+//   - the 35 flux modes are spread over the G lanes and summed across them;
 //   - lane j < 4 owns state component j of the DOP853 stages.
 // With G = 1 one lane does all of it (the reference's lane-per-binary layout).
 #pragma once
@@ -23,6 +23,7 @@ struct BinData {
     WaveCoef w;
     double eta_f;     // eta as the flux sees it (clamped to 1/4 within 1e-4)
     double nu;        // mu / M as RR_force computes it
+    double flux_scale;   // 1 / (8 pi nu): the flux sum's 1/(8 pi) and RR_force's 1/nu as one product
     PieceCoef pc;     // H's u-dependence in the common piece form (piece_setup)
 };
 
@@ -165,14 +166,16 @@ struct LanePair {
 // evaluated on different lanes of the group at once (lane q takes piece q; a
 // group of fewer than 13 lanes takes several per lane), gathered, and
 // combined in closed form on every lane.
+// For G >= 16, cf is this lane's piece (lane q < 13 takes piece q, the others
+// any piece); smaller groups read the pieces from bd.
 template <int G>
-__device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double pr, double L, int gl) {
+__device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double pr, double L, int gl,
+                                            const double (*cf)[3] = nullptr) {
     const double u = 1.0 / r, lr = log_pos(r), su = sqrt(u), hs = 0.5 * su * r;
     double v[kNumPieces], d[kNumPieces];
     if constexpr (G >= 16) {
-        const int q = gl < kNumPieces ? gl : 0;
         double pv, pd;
-        piece_eval(bd.pc.c[q], u, r, lr, su, hs, pv, pd);
+        piece_eval(cf, u, r, lr, su, hs, pv, pd);
 #pragma unroll
         for (int k = 0; k < kNumPieces; ++k) {
             v[k] = gread<G>(pv, k);
@@ -197,11 +200,12 @@ __device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double 
     else return ham_combine(bd.h, r, u, pr, L, v, d);
 }
 
-// Flux F (before the 1/nu and 1/Omega factors of RR_force), src/EOB.cu:2506-2560,
-// with the modes split over the group.  omega is the Omega the flux uses.
+// Flux sum 8 pi F (F before the 1/nu and 1/Omega factors of RR_force),
+// src/EOB.cu:2506-2560, with the modes split over the group.  omega is the Omega the flux uses.  For
+// G >= 35, zl is this lane's mode (lane i < 35 takes mode i, the others any mode).
 template <int G>
 __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, const BinData& bd, double r, double L, double Hreal,
-                                             double omega, double omc, int gl) {
+                                             double omega, double omc, int gl, const ModeCoef* zl = nullptr) {
     const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
     double v, cb_in;
     if constexpr (G >= 4) {
@@ -220,24 +224,41 @@ __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, co
     const double E1 = exp(-4.0 * kPi * Hreal * omega);
     double part = 0.0;
     if constexpr (G >= kNumModes) {
-        // one mode per lane; the spare lanes repeat the last mode and drop it
-        const double fm = flux_mode(mc[gl < kNumModes ? gl : kNumModes - 1], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+        // one mode per lane; the spare lanes evaluate some mode and drop it
+        const double fm = flux_mode(*zl, v, logv, Hreal, omega, E1, Heff, L, vPhi2);
         part = (gl < kNumModes) ? fm : 0.0;
     } else {
         for (int i = gl; i < kNumModes; i += G) part += flux_mode(mc[i], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
     }
-    return gsum<G>(part) / kPi / 8.0;
+    return gsum<G>(part);
 }
 
 // k = [dH/dpr, dH/dpphi, -dH/dr + Fr, Fphi] (src/EOB.cu:3047-3144), valid on every lane of the group.
 template <int G>
 __device__ __forceinline__ void group_rhs(const BinData& bd, const ModeCoef* __restrict__ mc, double r, double pr, double L, int gl,
                                           double k[4]) {
-    const HamPack p = ham_pack<G>(bd, r, pr, L, gl);
-    const double Hreal = bd.nu * p.H;
+    HamPack p;
+    double flux;
+    if constexpr (G >= kNumModes) {
+        // this lane's piece and mode coefficients are read before any arithmetic
+        // (the empty asm keeps the reads above it), so their latency overlaps
+        // the logarithm and square root that start H
+        double cf[kPieceDeg][3];
+        const double (*src)[3] = bd.pc.c[gl < kNumPieces ? gl : 0];
+#pragma unroll
+        for (int k = 0; k < kPieceDeg; ++k) {
+            cf[k][0] = src[k][0]; cf[k][1] = src[k][1]; cf[k][2] = src[k][2];
+        }
+        const ModeCoef zl = mc[gl < kNumModes ? gl : kNumModes - 1];
+        asm volatile("" ::: "memory");
+        p = ham_pack<G>(bd, r, pr, L, gl, cf);
+        flux = group_flux<G>(mc, bd, r, L, bd.nu * p.H, p.dHdL, p.omc, gl, &zl);
+    } else {
+        p = ham_pack<G>(bd, r, pr, L, gl, bd.pc.c[gl < kNumPieces ? gl : 0]);
+        flux = group_flux<G>(mc, bd, r, L, bd.nu * p.H, p.dHdL, p.omc, gl);
+    }
     const double omega = p.dHdL;   // the ODE overwrites omega with dH/dpphi before RR_force
-    double flux = group_flux<G>(mc, bd, r, L, Hreal, omega, p.omc, gl);
-    flux /= bd.nu;
+    flux *= bd.flux_scale;
     const double f_over_om = flux / omega;
     k[0] = p.dHdpr;
     k[1] = p.dHdL;

@@ -39,6 +39,7 @@ struct HamCoef {
     double e1, e2, as4, as5, sc6, bs3, bs4;
     // G[j][k]: coefficient of pr^(2j) u^k; logs at pr^0 u^4 and pr^2 u^3
     double g00, g01, g02, g03, g0l, g10, g11, g12, g13, g1l, g20, g21, g22, g30, g31, g40;
+    double M_nu;   // M / nu: H = (M / nu) sqrt(...) as one product
     double gs3, so4;
 };
 
@@ -54,6 +55,7 @@ EOB_HD void ham_setup(HamCoef& c, double m1, double m2, double chi1, double chi2
     const double s11 = c1 * c1 * X1 * X1, s22 = c2 * c2 * X2 * X2, s12 = c1 * c2 * nu;
     c.M = M;
     c.nu = nu;
+    c.M_nu = M / nu;
     c.ap2 = s11 + 2 * s12 + s22;
     const double ap = c1 * X1 + c2 * X2;
     c.apsq = ap * ap;
@@ -321,7 +323,7 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     const T dHe_dP = A * (Bnp + 1.0 + QP) * ihe + L * U3i * GtP;
     const T dHe_dL = A * L * W * iHe + U3i * Gt;
     HamPackT<T> p;
-    p.H = c.M * s / c.nu;
+    p.H = s * c.M_nu;
     p.dHdr = -u2 * kH * dHe_du;
     p.dHdpr = kH * dHe_dP * 2.0 * pr;
     p.dHdL = kH * dHe_dL;

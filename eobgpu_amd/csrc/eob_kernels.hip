@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_setup(const double* add, int n, 
         BinData d;
         ham_setup(d.h, m1, m2, c1, c2, add[5 * n + i], add[6 * n + i], add[7 * n + i], add[8 * n + i]);
         d.nu = nu_of(m1, m2);
+        d.flux_scale = 1.0 / (8.0 * kPi * d.nu);
         double ef = d.nu;
         if (ef > 0.25 && ef < 0.25 + 1e-4) ef = 0.25;
         d.eta_f = ef;
