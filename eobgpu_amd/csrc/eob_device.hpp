@@ -146,7 +146,8 @@ __device__ __forceinline__ double gread(double v, int q) {
     else return gshfl<G>(v, q);
 }
 
-// two square roots / reciprocals as one call on lanes 0 and 1 of the group
+// two square roots as one call on lanes 0 and 1 of the group; the refined
+// reciprocal is short enough that two of them cost less than the lane exchange
 template <int G>
 struct LanePair {
     int gl;
@@ -156,24 +157,47 @@ struct LanePair {
         y = gread<G>(t, 1);
     }
     __device__ void rcp2(double a, double b, double& x, double& y) const {
-        const double t = 1.0 / ((gl & 1) ? b : a);
-        x = gread<G>(t, 0);
-        y = gread<G>(t, 1);
+        x = qrcp(a);
+        y = qrcp(b);
     }
 };
+
+// Per-wavefront LDS slots for the piece gather of a 64-lane group: the 13
+// pieces' (value, d/du) and one slot the spare lanes write.  Every kernel that
+// integrates runs blocks of at most 256 threads (4 wavefronts).
+constexpr int kMaxWavesPerBlock = 4;
+__device__ __forceinline__ double2* piece_slots() {
+    __shared__ double2 slots[kMaxWavesPerBlock][kNumPieces + 1];
+    return slots[threadIdx.x >> 6];
+}
 
 // H and its derivatives at one state.  The 13 pieces of H's u-dependence are
 // evaluated on different lanes of the group at once (lane q takes piece q; a
 // group of fewer than 13 lanes takes several per lane), gathered, and
 // combined in closed form on every lane.
 // For G >= 16, cf is this lane's piece (lane q < 13 takes piece q, the others
-// any piece); smaller groups read the pieces from bd.
+// any piece); smaller groups read the pieces from pcs.
 template <int G>
-__device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double pr, double L, int gl,
+__device__ __forceinline__ HamPack ham_pack(const HamCoef& hc, const PieceCoef& pcs, double r, double pr, double L, int gl,
                                             const double (*cf)[3] = nullptr) {
-    const double u = 1.0 / r, lr = log_pos(r), su = sqrt(u), hs = 0.5 * su * r;
+    const double u = qrcp(r), lr = log_pos(r), su = sqrt(u), hs = 0.5 * su * r;
     double v[kNumPieces], d[kNumPieces];
-    if constexpr (G >= 16) {
+    if constexpr (G == 64) {
+        // gathered through this wavefront's LDS slots: one write, then 13
+        // same-address (broadcast) reads into vector registers; a wavefront's
+        // LDS accesses complete in order, so no barrier is involved
+        double pv, pd;
+        piece_eval(cf, u, r, lr, su, hs, pv, pd);
+        double2* slot = piece_slots();
+        slot[gl < kNumPieces ? gl : kNumPieces] = make_double2(pv, pd);
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < kNumPieces; ++k) {
+            const double2 t = slot[k];
+            v[k] = t.x;
+            d[k] = t.y;
+        }
+    } else if constexpr (G >= 16) {
         double pv, pd;
         piece_eval(cf, u, r, lr, su, hs, pv, pd);
 #pragma unroll
@@ -188,7 +212,7 @@ __device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double 
         for (int t = 0; t < PER; ++t) {
             int q = gl + t * G;
             if (q >= kNumPieces) q = 0;
-            piece_eval(bd.pc.c[q], u, r, lr, su, hs, pv[t], pd[t]);
+            piece_eval(pcs.c[q], u, r, lr, su, hs, pv[t], pd[t]);
         }
 #pragma unroll
         for (int k = 0; k < kNumPieces; ++k) {
@@ -196,17 +220,17 @@ __device__ __forceinline__ HamPack ham_pack(const BinData& bd, double r, double 
             d[k] = gshfl<G>(pd[k / G], k % G);
         }
     }
-    if constexpr (G >= 2) return ham_combine(bd.h, r, u, pr, L, v, d, LanePair<G>{gl});
-    else return ham_combine(bd.h, r, u, pr, L, v, d);
+    if constexpr (G >= 2) return ham_combine(hc, r, u, pr, L, v, d, LanePair<G>{gl});
+    else return ham_combine(hc, r, u, pr, L, v, d);
 }
 
 // Flux sum 8 pi F (F before the 1/nu and 1/Omega factors of RR_force),
 // src/EOB.cu:2506-2560, with the modes split over the group.  omega is the Omega the flux uses.  For
 // G >= 35, zl is this lane's mode (lane i < 35 takes mode i, the others any mode).
 template <int G>
-__device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, const BinData& bd, double r, double L, double Hreal,
+__device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, double eta_f, double r, double L, double Hreal,
                                              double omega, double omc, int gl, const ModeCoef* zl = nullptr) {
-    const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
+    const double vPhiIn = qrcp(omc * omc * (r * r * r));
     double v, cb_in;
     if constexpr (G >= 4) {
         // the two cube roots in one call, on two lanes
@@ -220,7 +244,7 @@ __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, co
     const double logv = log_pos(v);
     const double vPhi = r * cb_in * omega;
     const double vPhi2 = vPhi * vPhi;
-    const double Heff = (Hreal * Hreal - 1.0) / (2.0 * bd.eta_f) + 1.0;
+    const double Heff = qdiv(Hreal * Hreal - 1.0, 2.0 * eta_f) + 1.0;
     const double E1 = exp(-4.0 * kPi * Hreal * omega);
     double part = 0.0;
     if constexpr (G >= kNumModes) {
@@ -238,11 +262,11 @@ template <int G>
 __device__ __forceinline__ void group_rhs(const BinData& bd, const ModeCoef* __restrict__ mc, double r, double pr, double L, int gl,
                                           double k[4]) {
     HamPack p;
-    double flux;
+    double flux, fscale;
     if constexpr (G >= kNumModes) {
-        // this lane's piece and mode coefficients are read before any arithmetic
-        // (the empty asm keeps the reads above it), so their latency overlaps
-        // the logarithm and square root that start H
+        // this lane's piece and mode coefficients and the binary's scalars are
+        // read before the piece gather writes LDS (the empty asm keeps the
+        // reads above it; otherwise they are read again after that write)
         double cf[kPieceDeg][3];
         const double (*src)[3] = bd.pc.c[gl < kNumPieces ? gl : 0];
 #pragma unroll
@@ -250,19 +274,23 @@ __device__ __forceinline__ void group_rhs(const BinData& bd, const ModeCoef* __r
             cf[k][0] = src[k][0]; cf[k][1] = src[k][1]; cf[k][2] = src[k][2];
         }
         const ModeCoef zl = mc[gl < kNumModes ? gl : kNumModes - 1];
+        const HamCoef hc = bd.h;
+        const double nu = bd.nu, eta_f = bd.eta_f;
+        fscale = bd.flux_scale;
         asm volatile("" ::: "memory");
-        p = ham_pack<G>(bd, r, pr, L, gl, cf);
-        flux = group_flux<G>(mc, bd, r, L, bd.nu * p.H, p.dHdL, p.omc, gl, &zl);
+        p = ham_pack<G>(hc, bd.pc, r, pr, L, gl, cf);
+        flux = group_flux<G>(mc, eta_f, r, L, nu * p.H, p.dHdL, p.omc, gl, &zl);
     } else {
-        p = ham_pack<G>(bd, r, pr, L, gl, bd.pc.c[gl < kNumPieces ? gl : 0]);
-        flux = group_flux<G>(mc, bd, r, L, bd.nu * p.H, p.dHdL, p.omc, gl);
+        p = ham_pack<G>(bd.h, bd.pc, r, pr, L, gl, bd.pc.c[gl < kNumPieces ? gl : 0]);
+        flux = group_flux<G>(mc, bd.eta_f, r, L, bd.nu * p.H, p.dHdL, p.omc, gl);
+        fscale = bd.flux_scale;
     }
     const double omega = p.dHdL;   // the ODE overwrites omega with dH/dpphi before RR_force
-    flux *= bd.flux_scale;
-    const double f_over_om = flux / omega;
+    flux *= fscale;
+    const double f_over_om = qdiv(flux, omega);
     k[0] = p.dHdpr;
     k[1] = p.dHdL;
-    k[2] = -p.dHdr + (-pr / L * f_over_om);
+    k[2] = -p.dHdr + (qdiv(-pr, L) * f_over_om);
     k[3] = -0.0 + (-f_over_om);
 }
 

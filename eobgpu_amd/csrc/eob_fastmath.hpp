@@ -1,4 +1,4 @@
-// Logarithm for the integrator's right-hand side.
+// Logarithm and division for the integrator's right-hand side.
 //
 // The right-hand side runs as one wavefront per binary, so its cost is the
 // length of its instruction stream.  The device library's log carries a
@@ -57,6 +57,38 @@ EOB_HD double log_pos(double x) {
     const double t = fma(s, fma(R, z, hfsq), dk * kLn2Lo);
     const double res = fma(dk, kLn2Hi, -((hfsq - t) - f));
     return (x > 0.0 && x < HUGE_VAL) ? res : NAN;
+}
+
+// a / b and 1 / b for normal, finite operands with a normal result: the
+// hardware reciprocal estimate refined by two Newton steps, then one residual
+// correction (Markstein's method).  The quotient rounds as IEEE division on
+// such operands; the library's division adds scaling steps for the extreme
+// exponent ranges that these arguments never reach.  Other types divide.
+EOB_HD double rcp_estimate(double b) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcp(b);
+#else
+    return 1.0 / b;
+#endif
+}
+EOB_HD double refined_rcp(double b) {
+    double y = rcp_estimate(b);
+    y = fma(y, fma(-b, y, 1.0), y);
+    y = fma(y, fma(-b, y, 1.0), y);
+    return y;
+}
+template <class T>
+EOB_HD T qdiv(T a, T b) { return a / b; }
+template <class T>
+EOB_HD T qrcp(T b) { return 1.0 / b; }
+EOB_HD double qdiv(double a, double b) {
+    const double y = refined_rcp(b);
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+}
+EOB_HD double qrcp(double b) {
+    const double y = refined_rcp(b);
+    return fma(fma(-b, y, 1.0), y, y);
 }
 
 }  // namespace eob

@@ -307,11 +307,12 @@ EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T E1, T Heff, T
     const T hk = H * (double)z.m * Omega;
     const T x = 4.0 * kPi * hk;
     const T Em = ipow15(E1, z.m);   // m, l <= 8 and l + eps <= 9
-    T prod = 1.0;
     const T hk4 = 4.0 * hk * hk;
+    T prod = hk4 + 1.0;   // every mode has l >= 2
+    prod = prod * (hk4 + 4.0);
 #pragma unroll
-    for (int i = 1; i <= 8; ++i) prod = (i <= z.l) ? prod * (hk4 + (double)(i * i)) : prod;
-    const T T2 = x / (1.0 - Em) * prod * z.inv_lfac2;
+    for (int i = 3; i <= 8; ++i) prod = (i <= z.l) ? prod * (hk4 + (double)(i * i)) : prod;
+    const T T2 = qdiv(x, T(1.0) - Em) * prod * z.inv_lfac2;
     const T el = kEulerGamma + z.log2m + logv;
     // Horner over v^12 .. v^1
     T acc = z.c[10];

@@ -274,7 +274,7 @@ struct SerialPair {
     template <class T>
     EOB_HD void sqrt2(T a, T b, T& x, T& y) const { x = sqrt(a); y = sqrt(b); }
     template <class T>
-    EOB_HD void rcp2(T a, T b, T& x, T& y) const { x = 1.0 / a; y = 1.0 / b; }
+    EOB_HD void rcp2(T a, T b, T& x, T& y) const { x = qrcp(a); y = qrcp(b); }
 };
 
 // H, dH/dr, dH/dpr, dH/dpphi and dH/dpphi at pr = 0 from the piece values and
@@ -285,7 +285,7 @@ template <class T, class Pair = SerialPair>
 EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v, const T* d, const Pair& pair = Pair()) {
     (void)r;
     const T u2 = u * u, u3 = u2 * u, P = pr * pr, L2 = L * L;
-    const T iDn = 1.0 / v[kDn];
+    const T iDn = qrcp(v[kDn]);
     const T diDn = -d[kDn] * iDn * iDn;
     const T f1 = 1.0 - c.e1 * u, df1 = -c.e1;
     const T fe = f1 + c.e2 * u2, dfe = -c.e1 + 2.0 * c.e2 * u;

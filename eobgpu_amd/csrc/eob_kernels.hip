@@ -36,6 +36,7 @@ struct eobhip_seobnr {
 namespace {
 
 constexpr int kBlock = 256;
+static_assert(kBlock / 64 <= kMaxWavesPerBlock, "the piece-gather slots hold kMaxWavesPerBlock wavefronts");
 
 __device__ __forceinline__ HamCoef ham_from_add(const double* add, int n, int i) {
     HamCoef c;
