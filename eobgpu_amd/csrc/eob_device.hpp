@@ -84,6 +84,14 @@ __device__ __forceinline__ int gshfl_i(int v, int src) {
     if constexpr (G == 1) return v;
     else return __shfl(v, src, G);
 }
+// the value of lane `lane` of the wavefront (a scalar register read)
+__device__ __forceinline__ double read_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // lane permutation of a double within rows of 16 lanes (DPP)
 template <int CTRL>
 __device__ __forceinline__ double dpp_perm(double v) {
@@ -93,23 +101,29 @@ __device__ __forceinline__ double dpp_perm(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
+// lane permutation of a double across rows (DPP row broadcasts); rows outside
+// ROWS keep 0
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_bcast(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // Sum over the 64 lanes of the wavefront, the same on every lane: xor-1 and
 // xor-2 swaps inside quads, the half-row and row mirrors give every lane its
-// row's sum; the four row sums are then read out and added.
+// row's sum; row_bcast:15 adds row 0 into row 1 and row 2 into row 3,
+// row_bcast:31 adds rows 0+1 into row 3, and lane 63's total is read out:
+// (r3 + r2) + (r1 + r0).
 __device__ __forceinline__ double wave_sum(double v) {
     v += dpp_perm<0xB1>(v);    // quad_perm [1,0,3,2]
     v += dpp_perm<0x4E>(v);    // quad_perm [2,3,0,1]
     v += dpp_perm<0x141>(v);   // row_half_mirror
     v += dpp_perm<0x140>(v);   // row_mirror
-    const long long b = __double_as_longlong(v);
-    double r[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), 16 * q);
-        const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 16 * q);
-        r[q] = __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-    }
-    return (r[0] + r[1]) + (r[2] + r[3]);
+    v += dpp_bcast<0x142, 0xa>(v);   // row_bcast:15 into rows 1 and 3
+    v += dpp_bcast<0x143, 0xc>(v);   // row_bcast:31 into rows 2 and 3
+    return read_lane(v, 63);
 }
 
 template <int G>
@@ -131,14 +145,6 @@ __device__ __forceinline__ Dual ham_tangent(const HamCoef& hc, double r, double 
     return ham_eval<Dual>(hc, R, PR, LL);
 }
 
-// the value of lane `lane` of the wavefront (a scalar register read)
-__device__ __forceinline__ double read_lane(double v, int lane) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-
 // lane q of the group
 template <int G>
 __device__ __forceinline__ double gread(double v, int q) {
@@ -146,20 +152,18 @@ __device__ __forceinline__ double gread(double v, int q) {
     else return gshfl<G>(v, q);
 }
 
-// two square roots as one call on lanes 0 and 1 of the group; the refined
-// reciprocal is short enough that two of them cost less than the lane exchange
-template <int G>
-struct LanePair {
+// For a 64-lane group: lane 63, which owns no state component, no flux mode
+// and no piece, evaluates H's expression at P = pr^2 = 0, and dH/dpphi at
+// pr = 0 is read from it; the other lanes evaluate it at the state's pr.
+// Lane 63's H and derivatives therefore differ from the others', and the
+// integrator reads group-wide values (k1 for the stopping criterion) from
+// lane 0.
+struct LaneP0 {
+    static constexpr bool kLaneP0 = true;
+    static constexpr int kLane = 63;
     int gl;
-    __device__ void sqrt2(double a, double b, double& x, double& y) const {
-        const double t = sqrt((gl & 1) ? b : a);
-        x = gread<G>(t, 0);
-        y = gread<G>(t, 1);
-    }
-    __device__ void rcp2(double a, double b, double& x, double& y) const {
-        x = qrcp(a);
-        y = qrcp(b);
-    }
+    __device__ double P(double p) const { return gl == kLane ? 0.0 : p; }
+    __device__ double omc(double dHdL) const { return read_lane(dHdL, kLane); }
 };
 
 // Per-wavefront LDS slots for the piece gather of a 64-lane group: the 13
@@ -168,6 +172,13 @@ struct LanePair {
 constexpr int kMaxWavesPerBlock = 4;
 __device__ __forceinline__ double2* piece_slots() {
     __shared__ double2 slots[kMaxWavesPerBlock][kNumPieces + 1];
+    return slots[threadIdx.x >> 6];
+}
+
+// Per-wavefront LDS copy of the last first-stage k of a 64-lane group (a
+// rejected step retries from the same state, whose k1 is then not recomputed).
+__device__ __forceinline__ double* k1_slots() {
+    __shared__ double slots[kMaxWavesPerBlock][4];
     return slots[threadIdx.x >> 6];
 }
 
@@ -180,7 +191,7 @@ __device__ __forceinline__ double2* piece_slots() {
 template <int G>
 __device__ __forceinline__ HamPack ham_pack(const HamCoef& hc, const PieceCoef& pcs, double r, double pr, double L, int gl,
                                             const double (*cf)[3] = nullptr) {
-    const double u = qrcp(r), lr = log_pos(r), su = sqrt(u), hs = 0.5 * su * r;
+    const double u = qrcp(r), lr = log_pos(r), su = qsqrt(u), hs = 0.5 * su * r;
     double v[kNumPieces], d[kNumPieces];
     if constexpr (G == 64) {
         // gathered through this wavefront's LDS slots: one write, then 13
@@ -220,7 +231,7 @@ __device__ __forceinline__ HamPack ham_pack(const HamCoef& hc, const PieceCoef& 
             d[k] = gshfl<G>(pd[k / G], k % G);
         }
     }
-    if constexpr (G >= 2) return ham_combine(hc, r, u, pr, L, v, d, LanePair<G>{gl});
+    if constexpr (G == 64) return ham_combine(hc, r, u, pr, L, v, d, LaneP0{gl});
     else return ham_combine(hc, r, u, pr, L, v, d);
 }
 
@@ -245,14 +256,13 @@ __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, do
     const double vPhi = r * cb_in * omega;
     const double vPhi2 = vPhi * vPhi;
     const double Heff = qdiv(Hreal * Hreal - 1.0, 2.0 * eta_f) + 1.0;
-    const double E1 = exp(-4.0 * kPi * Hreal * omega);
     double part = 0.0;
     if constexpr (G >= kNumModes) {
         // one mode per lane; the spare lanes evaluate some mode and drop it
-        const double fm = flux_mode(*zl, v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+        const double fm = flux_mode(*zl, v, logv, Hreal, omega, Heff, L, vPhi2);
         part = (gl < kNumModes) ? fm : 0.0;
     } else {
-        for (int i = gl; i < kNumModes; i += G) part += flux_mode(mc[i], v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+        for (int i = gl; i < kNumModes; i += G) part += flux_mode(mc[i], v, logv, Hreal, omega, Heff, L, vPhi2);
     }
     return gsum<G>(part);
 }

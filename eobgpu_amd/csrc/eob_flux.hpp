@@ -300,13 +300,15 @@ EOB_HD void flux_setup(ModeCoef* mc, WaveCoef& wc, double m1, double m2, double 
 }
 
 // m^2 Omega^2 |h_lm|^2 for one mode (the flux summand before the 1/(8 pi) factor).
-//   E1 = exp(-4 pi H Omega), Heff = (H^2-1)/(2 eta)+1, vPhi2 = (vPhi)^2
+//   Heff = (H^2-1)/(2 eta)+1, vPhi2 = (vPhi)^2; the tail factor's exp(-4 pi m H Omega)
+//   is evaluated per mode (the reference raises exp(-4 pi H Omega) to the m-th power,
+//   the same number within rounding)
 // (templated on the scalar so the FP64 operation count can be taken with a counting type)
 template <class T>
-EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T E1, T Heff, T pphi, T vPhi2) {
+EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T Heff, T pphi, T vPhi2) {
     const T hk = H * (double)z.m * Omega;
     const T x = 4.0 * kPi * hk;
-    const T Em = ipow15(E1, z.m);   // m, l <= 8 and l + eps <= 9
+    const T Em = exp(-x);
     const T hk4 = 4.0 * hk * hk;
     T prod = hk4 + 1.0;   // every mode has l >= 2
     prod = prod * (hk4 + 4.0);
@@ -329,7 +331,7 @@ EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T E1, T Heff, T
     acc = acc * v + z.c[0];
     const T rho = 1.0 + acc * v;
     const T aux = v * (z.f1 + v * v * z.f3);
-    T rpl = ipow15(rho, z.l);
+    T rpl = ipow15(rho, z.l);   // l <= 8 and l + eps <= 9
     rpl = (z.flags & 2) ? aux : rpl + aux;
     const T S = (z.flags & 1) ? Heff : v * pphi;
     const T hN2 = z.N2 * ipow15(vPhi2, z.lpe);

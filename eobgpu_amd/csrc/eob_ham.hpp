@@ -268,23 +268,23 @@ struct HamPackT {
 };
 using HamPack = HamPackT<double>;
 
-// Two independent square roots or reciprocals.  On one lane they are two
-// calls; a lane group (eob_device.hpp) evaluates them as one call on two lanes.
-struct SerialPair {
-    template <class T>
-    EOB_HD void sqrt2(T a, T b, T& x, T& y) const { x = sqrt(a); y = sqrt(b); }
-    template <class T>
-    EOB_HD void rcp2(T a, T b, T& x, T& y) const { x = qrcp(a); y = qrcp(b); }
+// Where dH/dpphi at pr = 0 (the circular frequency) comes from: SeparateP0
+// evaluates that branch beside the full one on the same lane; a wavefront can
+// instead let one of its lanes evaluate the whole expression at pr = 0 (LaneP0
+// in eob_device.hpp), since it is the same expression with P = 0.
+struct SeparateP0 {
+    static constexpr bool kLaneP0 = false;
 };
 
 // H, dH/dr, dH/dpr, dH/dpphi and dH/dpphi at pr = 0 from the piece values and
 // u-derivatives (v, d indexed by HamPiece); the closed-form derivatives of
-// ham_eval's expression.  The pr = 0 branch (for the circular frequency) runs
-// beside the full one, its square roots and reciprocals paired with theirs.
-template <class T, class Pair = SerialPair>
-EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v, const T* d, const Pair& pair = Pair()) {
+// ham_eval's expression.
+template <class T, class Br = SeparateP0>
+EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v, const T* d, const Br& br = Br()) {
     (void)r;
-    const T u2 = u * u, u3 = u2 * u, P = pr * pr, L2 = L * L;
+    T P = pr * pr;
+    if constexpr (Br::kLaneP0) P = br.P(P);
+    const T u2 = u * u, u3 = u2 * u, L2 = L * L;
     const T iDn = qrcp(v[kDn]);
     const T diDn = -d[kDn] * iDn * iDn;
     const T f1 = 1.0 - c.e1 * u, df1 = -c.e1;
@@ -305,19 +305,16 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     const T dQ = P2 * (d[kQ4] + P * (d[kQ6] + P * d[kQ8]));
     const T QP = P * (2.0 * v[kQ4] + P * (3.0 * v[kQ6] + 4.0 * P * v[kQ8]));   // dQ/dP
     const T X = (Bnp + 1.0) * P + W * L2 + Q + 1.0;
-    T Heven, He0, iHe, iHe0;
-    pair.sqrt2(A * X, A * (W * L2 + 1.0), Heven, He0);
-    pair.rcp2(Heven, He0, iHe, iHe0);
+    const T Heven = qsqrt(A * X);
+    const T iHe = qrcp(Heven);
     const T ihe = 0.5 * iHe;
     const T Gt = v[kG0] + P * (v[kG1] + P * (v[kG2] + P * (v[kG3] + P * c.g40))) + v[kGS];
     const T dGt = d[kG0] + P * (d[kG1] + P * (d[kG2] + P * d[kG3])) + d[kGS];
     const T GtP = v[kG1] + P * (2.0 * v[kG2] + P * (3.0 * v[kG3] + 4.0 * P * c.g40));
     const T U3i = u3 * iDn, dU3i = 3.0 * u2 * iDn + u3 * diDn;
     const T Heff = Heven + L * U3i * Gt;
-    const T Gt0 = v[kG0] + v[kGS];
-    T s, s0, is, is0;
-    pair.sqrt2(1.0 + 2.0 * c.nu * (Heff - 1.0), 1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0), s, s0);
-    pair.rcp2(s, s0, is, is0);
+    const T s = qsqrt(1.0 + 2.0 * c.nu * (Heff - 1.0));
+    const T is = qrcp(s);
     const T kH = c.M * is;   // dH/dHeff
     const T dHe_du = (dA * X + A * (dBnp * P + dW * L2 + dQ)) * ihe + L * (dU3i * Gt + U3i * dGt);
     const T dHe_dP = A * (Bnp + 1.0 + QP) * ihe + L * U3i * GtP;
@@ -327,13 +324,20 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     p.dHdr = -u2 * kH * dHe_du;
     p.dHdpr = kH * dHe_dP * 2.0 * pr;
     p.dHdL = kH * dHe_dL;
-    p.omc = (c.M * is0) * (A * L * W * iHe0 + U3i * Gt0);
+    if constexpr (Br::kLaneP0) {
+        p.omc = br.omc(p.dHdL);
+    } else {
+        const T He0 = qsqrt(A * (W * L2 + 1.0));
+        const T Gt0 = v[kG0] + v[kGS];
+        const T s0 = qsqrt(1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0));
+        p.omc = (c.M * qrcp(s0)) * (A * L * W * qrcp(He0) + U3i * Gt0);
+    }
     return p;
 }
 
 // serial form (one lane evaluates all pieces)
 EOB_HD HamPack ham_pack_serial(const HamCoef& c, const PieceCoef& pc, double r, double pr, double L) {
-    const double u = 1.0 / r, lr = log_pos(r), su = sqrt(u), hs = 0.5 * su * r;
+    const double u = qrcp(r), lr = log_pos(r), su = qsqrt(u), hs = 0.5 * su * r;
     double v[kNumPieces], d[kNumPieces];
     for (int q = 0; q < kNumPieces; ++q) piece_eval(pc.c[q], u, r, lr, su, hs, v[q], d[q]);
     return ham_combine(c, r, u, pr, L, v, d);

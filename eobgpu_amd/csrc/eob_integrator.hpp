@@ -65,6 +65,7 @@ struct GState {
     int b, pass, step_num, sn1, code;
     int attempted, just_acc, stop_flag, prev_reject;
     int fail;                    // 1: non-finite error or step, 2: attempt cap reached
+    int k1_ok;                   // 64-lane groups: k1 of the current state is in k1_slots()
     double x, h, err_old, h_init;
     double om0, om1, om2, om3;   // last four stored omegas, om3 newest
     double y[NC];                // owned state components
@@ -124,6 +125,7 @@ __device__ __forceinline__ void start_pass(GState<G>& s, int pass, double h0) {
     s.stop_flag = 1;
     s.code = 0;
     s.om0 = s.om1 = s.om2 = s.om3 = 0.0;
+    s.k1_ok = 0;
 }
 
 template <int G>
@@ -327,6 +329,7 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
     if (!(err >= 0.0) || !(fabs(s.h) < 1e300) || s.h == 0.0) s.fail = 1;
     else if (s.att >= P.max_attempts) s.fail = 2;
     if (ok) {
+        s.k1_ok = 0;
         s.x = s.x + h_old;
 #pragma unroll
         for (int q = 0; q < NC; ++q) s.y[q] = ynew[q];
@@ -352,7 +355,22 @@ __device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int 
     double yf[4];
     gather_state<G>(s, gl, yf);
     double k1[4];
-    group_rhs<G>(bd, mc, yf[0], yf[2], yf[3], gl, k1);
+    if constexpr (G == 64) {
+        // after a rejected attempt the state is unchanged, and so is its k1
+        double* k1s = k1_slots();
+        if (s.k1_ok) {
+            k1[0] = k1s[0]; k1[1] = k1s[1]; k1[2] = k1s[2]; k1[3] = k1s[3];
+        } else {
+            group_rhs<G>(bd, mc, yf[0], yf[2], yf[3], gl, k1);
+            // lane 63 evaluated H at pr = 0 (LaneP0): the group's k1 is lane 0's
+#pragma unroll
+            for (int c = 0; c < 4; ++c) k1[c] = read_lane(k1[c], 0);
+            k1s[0] = k1[0]; k1s[1] = k1[1]; k1s[2] = k1[2]; k1s[3] = k1[3];   // every lane stores the same values
+            s.k1_ok = 1;
+        }
+    } else {
+        group_rhs<G>(bd, mc, yf[0], yf[2], yf[3], gl, k1);
+    }
     if (s.attempted) {
         if (s.just_acc) {
             // StoppingCriterion at the accepted state (eob/seobnrv4phm.py:82-150)

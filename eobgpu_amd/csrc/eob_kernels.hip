@@ -119,7 +119,6 @@ __device__ double flux_dropin(const ModeCoef* hm, const double* add, int n, int 
     const double vPhi = r * cbrt(vPhiIn) * omega;
     const double vPhi2 = vPhi * vPhi;
     const double Heff = (Hreal * Hreal - 1.0) / (2.0 * eta_f) + 1.0;
-    const double E1 = exp(-4.0 * kPi * Hreal * omega);
     const double h22c = add[9 * n + i];
     double sum = 0.0;
     for (int mi = 0; mi < kNumModes; ++mi) {
@@ -131,7 +130,7 @@ __device__ double flux_dropin(const ModeCoef* hm, const double* add, int n, int 
         int fl = ((z.l + z.m) & 1) ? 0 : 1;
         if (eta_f == 0.25 && (z.m & 1)) fl |= 2;
         z.flags = fl;
-        sum += flux_mode(z, v, logv, Hreal, omega, E1, Heff, L, vPhi2);
+        sum += flux_mode(z, v, logv, Hreal, omega, Heff, L, vPhi2);
     }
     return sum / kPi / 8.0;
 }
@@ -429,6 +428,7 @@ __device__ __forceinline__ void load_state(const PlainState& S, GState<G>& s, in
     const int f = S.flags[b];
     s.attempted = f & 1; s.just_acc = (f >> 1) & 1; s.stop_flag = (f >> 2) & 1; s.prev_reject = (f >> 3) & 1;
     s.fail = (f >> 4) & 3;
+    s.k1_ok = 0;
     s.x = S.x[b]; s.h = S.h[b]; s.err_old = S.err_old[b]; s.h_init = S.h_init[b];
     s.om0 = S.om[b]; s.om1 = S.om[n + b]; s.om2 = S.om[2 * n + b]; s.om3 = S.om[3 * n + b];
     s.att = S.att[b];

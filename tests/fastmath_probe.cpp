@@ -1,8 +1,19 @@
-// Host build of the device path's log_pos (eobgpu_amd/csrc/eob_fastmath.hpp):
-// the header writes every multiply-add as fma() and switches contraction off,
-// so this build rounds exactly as the device build does.
+// Host build of the right-hand side's arithmetic helpers
+// (eobgpu_amd/csrc/eob_fastmath.hpp).  log_pos writes every multiply-add as
+// fma() with contraction off, so this build rounds as the device build does;
+// the division and square-root helpers start here from single-precision
+// estimates (coarser than the hardware's), so this build tests their refinement.
 #include "../eobgpu_amd/csrc/eob_fastmath.hpp"
 
 extern "C" void probe_log_pos(const double* x, double* y, int n) {
     for (int i = 0; i < n; ++i) y[i] = eob::log_pos(x[i]);
+}
+extern "C" void probe_qdiv(const double* a, const double* b, double* y, int n) {
+    for (int i = 0; i < n; ++i) y[i] = eob::qdiv(a[i], b[i]);
+}
+extern "C" void probe_qrcp(const double* b, double* y, int n) {
+    for (int i = 0; i < n; ++i) y[i] = eob::qrcp(b[i]);
+}
+extern "C" void probe_qsqrt(const double* x, double* y, int n) {
+    for (int i = 0; i < n; ++i) y[i] = eob::qsqrt(x[i]);
 }

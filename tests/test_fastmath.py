@@ -1,8 +1,12 @@
-"""The right-hand side's logarithm (eob_fastmath.hpp, log_pos) against
-correctly rounded logarithms from Python's decimal module: within 1 ulp over
-the radii and frequencies the integrator sees and over the whole exponent
-range; NaN outside the positive finite numbers.  The header is compiled for
-the host with contraction off, which rounds as the device build does."""
+"""The right-hand side's arithmetic helpers (eob_fastmath.hpp) on the CPU.
+
+log_pos against correctly rounded logarithms from Python's decimal module:
+within 1 ulp over the radii and frequencies the integrator sees and over the
+whole exponent range; NaN outside the positive finite numbers (the header is
+compiled with contraction off, which rounds as the device build does).
+qdiv / qrcp / qsqrt, started from single-precision estimates, against IEEE
+division and square root (Python's, correctly rounded): within 1 ulp, and
+equal in all but a handful of cases."""
 import ctypes
 import math
 import os
@@ -25,12 +29,18 @@ def probe(tmp_path_factory):
     subprocess.run([cxx, "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", out,
                     os.path.join(HERE, "fastmath_probe.cpp")], check=True)
     lib = ctypes.CDLL(out)
-    lib.probe_log_pos.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    for name in ("probe_log_pos", "probe_qrcp", "probe_qsqrt"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    lib.probe_qdiv.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int]
 
-    def run(x):
+    def run(x, fn="probe_log_pos", b=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
-        lib.probe_log_pos(x.ctypes.data, y.ctypes.data, len(x))
+        if b is None:
+            getattr(lib, fn)(x.ctypes.data, y.ctypes.data, len(x))
+        else:
+            b = np.ascontiguousarray(b, dtype=np.float64)
+            getattr(lib, fn)(x.ctypes.data, b.ctypes.data, y.ctypes.data, len(x))
         return y
     return run
 
@@ -64,3 +74,18 @@ def test_within_one_ulp(probe):
 def test_non_positive_and_non_finite(probe):
     y = probe(np.array([0.0, -0.0, -1.0, -np.inf, np.inf, np.nan]))
     assert np.all(np.isnan(y))
+
+
+def _ulps(got, ref):
+    return np.abs(got - ref) / np.spacing(np.abs(ref))
+
+
+def test_division_and_square_root(probe):
+    rng = np.random.default_rng(12)
+    a = rng.choice([-1.0, 1.0], 50000) * 10.0 ** rng.uniform(-8, 8, 50000)
+    b = 10.0 ** rng.uniform(-8, 8, 50000)
+    for got, ref in ((probe(a, "probe_qdiv", b), a / b), (probe(b, "probe_qrcp"), 1.0 / b),
+                     (probe(b, "probe_qsqrt"), np.sqrt(b))):
+        u = _ulps(got, ref)
+        assert u.max() <= 1.0
+        assert np.count_nonzero(got != ref) <= 5
