@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 # FP64 operations per attempted DOP853 step (12 right-hand sides + stages +
 # error norm + controller), counted by eobgpu_amd/csrc/flop_count.cpp on the
 # library's own Hamiltonian and flux code.
-STEP_FLOPS = 49691
+STEP_FLOPS = 48395
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--chunks", type=int, default=4,
+                    help="the headline pass splits the batch into this many chunks of similar expected length on their "
+                         "own streams (eobgpu_amd.pipeline.ChunkedPass); 1 times the one-batch pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
     ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
@@ -149,6 +152,30 @@ def time_pipeline(pipe, mode, group, steps, warmup, direct=False, block=256, gri
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
     return wall, {k: v / steps for k, v in stage.items()}
+
+
+def time_chunked(pipe, chunks, steps, warmup, group=64, block=64):
+    """The same pass as Pipeline.run (support-length likelihood) with the batch in
+    `chunks` stream-overlapped chunks; returns (wall per pass, log L, trajectory
+    time of the longest chunk)."""
+    torch, P = pipe.torch, pipe.P
+    p = pipe.params
+    cp = P.ChunkedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], pipe.lik, pipe.M, pipe.dt, amp=pipe.amp,
+                       max_step=pipe.b.max_step, chunks=chunks, rtol=pipe.rtol)
+    for _ in range(warmup):
+        cp.run(group=group, block=block)
+    torch.cuda.synchronize()
+    t_traj = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev = {"start": torch.cuda.Event(enable_timing=True), "traj_end_longest": torch.cuda.Event(enable_timing=True)}
+        ev["start"].record()
+        cp.run(group=group, block=block, events=ev)
+        torch.cuda.synchronize()
+        t_traj += ev["start"].elapsed_time(ev["traj_end_longest"])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, cp.out.cpu().numpy(), t_traj / steps, [len(g) for g in cp.groups]
 
 
 def cpu_baseline(pipe, nb):
@@ -404,6 +431,16 @@ def main():
                    "max_rel_diff_logl": float(np.max(np.abs(logl_support - logl_direct) / np.abs(logl_direct)))}
     print(f"[bench] full-length likelihood path: {json.dumps(full_length)}", file=sys.stderr, flush=True)
 
+    chunked = None
+    headline_wall = wall
+    if args.chunks > 1:
+        wall_c, logl_c, traj_c, sizes = time_chunked(pipe, args.chunks, args.steps, args.warmup, group=group)
+        chunked = {"chunks": args.chunks, "chunk_sizes": sizes, "ms_per_step": wall_c * 1e3,
+                   "trajectory_longest_chunk_ms": traj_c, "one_batch_ms_per_step": wall * 1e3,
+                   "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_c - logl_support) / np.abs(logl_support)))}
+        headline_wall = wall_c
+        print(f"[bench] chunked pass: {json.dumps(chunked)}", file=sys.stderr, flush=True)
+
     plain = None
     if not args.no_plain:
         # the reference's batching model: one launch per attempted step, one lane per
@@ -412,7 +449,7 @@ def main():
         for name, grp in (("lane_per_binary", args.plain_group), ("group64", 64)):
             wall_p, stages_p = time_pipeline(pipe, "plain", grp, args.plain_steps, 0)
             plain[name] = {"value": args.n / wall_p, "ms_per_step": wall_p * 1e3, "group": grp,
-                           "trajectory_ms": stages_p["trajectory"], "speedup_of_persistent": wall_p / wall}
+                           "trajectory_ms": stages_p["trajectory"], "speedup_of_persistent": wall_p / headline_wall}
             print(f"[bench] plain ({name}): {wall_p*1e3:.2f} ms/step", file=sys.stderr, flush=True)
 
     served = None
@@ -445,15 +482,15 @@ def main():
                "max_rel_diff_logl_vs_gpu": float(np.max(np.abs(logl_cpu - logl_support[:k]) / np.abs(logl_support[:k])))}
 
     out = {
-        "metric": "likelihood evals/s @1024 uneven binaries", "value": args.n / wall, "unit": "likelihood evals/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True,
+        "metric": "likelihood evals/s @1024 uneven binaries", "value": args.n / headline_wall, "unit": "likelihood evals/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": headline_wall * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[15,40] M; "
                                "two-pass DOP853 rtol=atol=1e-10; (2,2) on 2^20 samples at 1/4096 s; FD likelihood (support-length transforms)",
                    "global_batch": args.n, "seq_len": pipe.N, "parallelism": "none (1 GPU)", "group_lanes": group,
-                   "block": block, "grid_groups": grid},
+                   "block": block, "grid_groups": grid, "chunks": args.chunks},
         "autotune": None if tuned is None else {k: tuned[k] for k in ("group", "block", "grid_groups", "time_s", "table")},
-        "stages_ms": stages, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
+        "stages_ms": stages, "one_batch_ms_per_step": wall * 1e3, "chunked": chunked, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
         "attempted_steps_total": int(total_attempts), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
         "logl_finite": bool(np.all(np.isfinite(logl))),
         "full_length_likelihood": full_length, "plain_batch": plain, "served": served, "roofline": roof, "cpu_baseline": cpu,

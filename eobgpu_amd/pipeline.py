@@ -282,6 +282,83 @@ class SupportLikelihood:
         return out
 
 
+def expected_length(r0, m1, m2):
+    """A monotone proxy of a binary's number of integrator steps: orbits to merger
+    grow as r0^(5/2) / nu (leading post-Newtonian order), and the adaptive step is
+    a roughly fixed fraction of an orbit.  Used only to group binaries of similar
+    length; it changes no result."""
+    m1 = np.asarray(m1, dtype=np.float64)
+    m2 = np.asarray(m2, dtype=np.float64)
+    nu = m1 * m2 / (m1 + m2) ** 2
+    return np.asarray(r0, dtype=np.float64) ** 2.5 / nu
+
+
+class ChunkedPass:
+    """One pass over n binaries -- tables, initial conditions, the two-pass
+    trajectory, the (2,2) knots, the splines and the support-length likelihood
+    (``SupportLikelihood``) -- with the batch split into ``chunks`` groups of
+    binaries of similar expected length, each with its own buffers and stream.
+
+    Every chunk's persistent integrator starts at once, so together they occupy
+    the SIMDs as one launch would.  The chunks of shorter binaries finish first
+    and run their knots, splines, transforms and sums while the longer chunks
+    are still integrating: that downstream work fills the integrator's tail,
+    in which only the longest binaries' wavefronts still run.  Each binary's
+    numbers are those of the one-batch pass (the kernels treat binaries
+    independently; the transforms run on other batch sizes, so log L may
+    differ in the last bits).
+    """
+
+    def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, chunks=4, rtol=1e-10,
+                 device="cuda", order=None):
+        m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
+        n = m1.size
+        if chunks < 1 or chunks > n:
+            raise ValueError("chunks must be in [1, n]")
+        if order is None:
+            order = np.argsort(-expected_length(r0, m1, m2), kind="stable")   # longest first
+        self.n, self.lik, self.dt, self.rtol, self.device = n, lik, float(dt), float(rtol), device
+        M = np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))
+        A = np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (n,))
+        self.groups = [np.asarray(g) for g in np.array_split(np.asarray(order), chunks)]
+        self.streams = [torch.cuda.Stream(device=device) for _ in self.groups]
+        self.batches, self.r0, self.M, self.amp, self.idx, self.outs = [], [], [], [], [], []
+        for g in self.groups:
+            self.batches.append(EOBBatch(make_additional_args(m1[g], m2[g], chi1[g], chi2[g]), max_step=max_step, device=device))
+            self.r0.append(r0[g].copy())
+            self.M.append(M[g].copy())
+            self.amp.append(A[g].copy())
+            self.idx.append(torch.as_tensor(g, dtype=torch.int64, device=device))
+            self.outs.append(torch.empty(g.size, dtype=torch.float64, device=device))
+        self.out = torch.empty(n, dtype=torch.float64, device=device)
+        torch.cuda.synchronize(device)
+
+    def run(self, group=64, block=64, grid_groups=0, events=None):
+        """Launch the pass; returns log L [n] in the caller's order (valid on the
+        current stream once it returns).  ``events`` (optional) gets
+        'traj_end_longest' recorded when the longest chunk's trajectory ends."""
+        cur = torch.cuda.current_stream(self.device)
+        for st in self.streams:
+            st.wait_stream(cur)
+        for k, (b, st) in enumerate(zip(self.batches, self.streams)):   # longest first
+            with torch.cuda.stream(st):
+                call("eobhip_bin_setup", b.add.data_ptr(), b.n, 1, b.bindata.data_ptr(), b.modes.data_ptr(), current_stream())
+                b.initial_conditions(self.r0[k])
+                b.integrate(rtol=self.rtol, atol=self.rtol, two_pass=True, group=group, block=block, grid_groups=grid_groups)
+                if k == 0 and events is not None:
+                    events["traj_end_longest"].record(st)
+        for k in reversed(range(len(self.batches))):   # shortest first: they finish first
+            b, st = self.batches[k], self.streams[k]
+            with torch.cuda.stream(st):
+                b.hlm22_knots(out=getattr(b, "knots", None))
+                b.build_splines()
+                self.lik(b, self.M[k], self.dt, amp=self.amp[k], out=self.outs[k])
+                self.out.index_copy_(0, self.idx[k], self.outs[k])
+        for st in self.streams:
+            cur.wait_stream(st)
+        return self.out
+
+
 def analytic_psd(f, f0=200.0):
     """S(f) = (f/f0)^-4 + 1 + (f/f0)^2 (S(0) = inf)."""
     x = np.asarray(f, dtype=np.float64) / f0
