@@ -232,6 +232,30 @@ class SupportLikelihood:
         call("eobhip_lik_setup", d.data_ptr(), self.N, self.df, self.f0, self.m_min, self.g.data_ptr(), self.dd.data_ptr(),
              self.bank.data_ptr(), scratch.data_ptr(), current_stream())
         self._keep = (d, scratch)
+        self._work = {}
+
+    def _buffers(self, n, ph, px, gx, gx2):
+        """Work buffers of the calls on the current stream, kept between calls and
+        grown when a call needs more (calls on different streams never share
+        them, so chunks of one pass can run concurrently)."""
+        key = torch.cuda.current_stream(self.device).cuda_stream
+        w = self._work.get(key)
+        need = {"n": n, "h": max(ph, 1), "x": max(px, 1), "cross": n * gx, "hh": n * gx2}
+        if w is None or any(w["cap"][k] < v for k, v in need.items()):
+            cap = {k: max(v, 0 if w is None else w["cap"][k]) for k, v in need.items()}
+            dev = self.device
+            w = {"cap": cap,
+                 "nsamp": torch.empty(cap["n"], dtype=torch.int32, device=dev),
+                 "mlog2": torch.empty(cap["n"], dtype=torch.int32, device=dev),
+                 "mlog2_host": torch.empty(cap["n"], dtype=torch.int32, pin_memory=True),
+                 "offs_host": torch.empty((2, cap["n"]), dtype=torch.int64, pin_memory=True),
+                 "offs": torch.empty((2, cap["n"]), dtype=torch.int64, device=dev),
+                 "hbuf": torch.empty(cap["h"], dtype=torch.float64, device=dev),
+                 "X": torch.empty(cap["x"], dtype=torch.complex128, device=dev),
+                 "cross": torch.empty(max(cap["cross"], 1), dtype=torch.float64, device=dev),
+                 "hh": torch.empty(max(cap["hh"], 1), dtype=torch.float64, device=dev)}
+            self._work[key] = w
+        return w
 
     def __call__(self, batch, total_mass_msun, dt, amp=None, out=None, terms=None):
         n, dev = batch.n, self.device
@@ -242,11 +266,14 @@ class SupportLikelihood:
             amp_t = torch.ones(n, dtype=torch.float64, device=dev)
         else:
             amp_t = torch.as_tensor(np.broadcast_to(np.asarray(amp, dtype=np.float64), (n,)).copy(), device=dev)
-        nsamp = torch.empty(n, dtype=torch.int32, device=dev)
-        mlog2 = torch.empty(n, dtype=torch.int32, device=dev)
+        w = self._buffers(n, 0, 0, 0, 0)
+        nsamp, mlog2 = w["nsamp"][:n], w["mlog2"][:n]
         call("eobhip_lik_sizes", batch.dense.data_ptr(), batch.num_steps.data_ptr(), dT_M.data_ptr(), n, batch.max_step, self.N,
              self.m_min, nsamp.data_ptr(), mlog2.data_ptr(), st)
-        ml = mlog2.cpu().numpy()
+        ml_host = w["mlog2_host"][:n]
+        ml_host.copy_(mlog2, non_blocking=True)
+        torch.cuda.current_stream(dev).synchronize()
+        ml = ml_host.numpy().copy()
         order = np.argsort(ml, kind="stable")
         rowoff = np.empty(n, dtype=np.int64)
         xoff = np.empty(n, dtype=np.int64)
@@ -261,13 +288,15 @@ class SupportLikelihood:
             ph += Mk * idx.size
             px += (Mk // 2 + 1) * idx.size
         kmax = int(ml.max()) if n else self.m_min
-        offs = torch.from_numpy(np.stack([rowoff, xoff])).to(dev)
-        hbuf = torch.empty(max(ph, 1), dtype=torch.float64, device=dev)
-        X = torch.empty(max(px, 1), dtype=torch.complex128, device=dev)
         gx, gx2 = ctypes.c_int(0), ctypes.c_int(0)
         call("eobhip_lik_partial_blocks", kmax, ctypes.byref(gx), ctypes.byref(gx2))
-        cross = torch.empty(n * gx.value, dtype=torch.float64, device=dev)
-        hh = torch.empty(n * gx2.value, dtype=torch.float64, device=dev)
+        w = self._buffers(n, ph, px, gx.value, gx2.value)
+        oh = w["offs_host"][:, :n]
+        oh[0].numpy()[:] = rowoff
+        oh[1].numpy()[:] = xoff
+        offs = w["offs"][:, :n]
+        offs.copy_(oh, non_blocking=True)
+        hbuf, X, cross, hh = w["hbuf"], w["X"], w["cross"], w["hh"]
         call("eobhip_lik_resample", batch.dense.data_ptr(), batch.coef.data_ptr(), batch.num_steps.data_ptr(), amp_t.data_ptr(),
              dT_M.data_ptr(), nsamp.data_ptr(), mlog2.data_ptr(), offs[0].data_ptr(), n, batch.max_step, kmax, self.g.data_ptr(),
              hbuf.data_ptr(), cross.data_ptr(), st)
@@ -278,7 +307,7 @@ class SupportLikelihood:
         call("eobhip_lik_finish", X.data_ptr(), offs[1].data_ptr(), mlog2.data_ptr(), n, kmax, self.m_min, self.bank.data_ptr(),
              cross.data_ptr(), hh.data_ptr(), self.dd.data_ptr(), self.df, out.data_ptr(), None if terms is None else terms.data_ptr(), st)
         self.nsamp, self.mlog2 = nsamp, mlog2
-        self._tmp = (dT_M, amp_t, offs, hbuf, X, cross, hh)
+        self._tmp = (dT_M, amp_t)
         return out
 
 
