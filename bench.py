@@ -58,9 +58,9 @@ def parse():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--chunks", type=int, default=4,
-                    help="the headline pass splits the batch into this many chunks of similar expected length on their "
-                         "own streams (eobgpu_amd.pipeline.ChunkedPass); 1 times the one-batch pass")
+    ap.add_argument("--chunk-fractions", default="0.1,0.3,0.6",
+                    help="the headline pass splits the batch into chunks of these shares (longest binaries first) on their "
+                         "own streams (eobgpu_amd.pipeline.ChunkedPass); '1' times the one-batch pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
     ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
@@ -154,14 +154,14 @@ def time_pipeline(pipe, mode, group, steps, warmup, direct=False, block=256, gri
     return wall, {k: v / steps for k, v in stage.items()}
 
 
-def time_chunked(pipe, chunks, steps, warmup, group=64, block=64):
+def time_chunked(pipe, fractions, steps, warmup, group=64, block=64):
     """The same pass as Pipeline.run (support-length likelihood) with the batch in
-    `chunks` stream-overlapped chunks; returns (wall per pass, log L, trajectory
-    time of the longest chunk)."""
+    stream-overlapped chunks of the given shares; returns (wall per pass, log L,
+    trajectory time of the longest chunk, chunk sizes)."""
     torch, P = pipe.torch, pipe.P
     p = pipe.params
     cp = P.ChunkedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], pipe.lik, pipe.M, pipe.dt, amp=pipe.amp,
-                       max_step=pipe.b.max_step, chunks=chunks, rtol=pipe.rtol)
+                       max_step=pipe.b.max_step, fractions=fractions, rtol=pipe.rtol)
     for _ in range(warmup):
         cp.run(group=group, block=block)
     torch.cuda.synchronize()
@@ -433,9 +433,10 @@ def main():
 
     chunked = None
     headline_wall = wall
-    if args.chunks > 1:
-        wall_c, logl_c, traj_c, sizes = time_chunked(pipe, args.chunks, args.steps, args.warmup, group=group)
-        chunked = {"chunks": args.chunks, "chunk_sizes": sizes, "ms_per_step": wall_c * 1e3,
+    fractions = [float(x) for x in args.chunk_fractions.split(",") if x.strip()]
+    if len(fractions) > 1:
+        wall_c, logl_c, traj_c, sizes = time_chunked(pipe, fractions, args.steps, args.warmup, group=group)
+        chunked = {"fractions": fractions, "chunk_sizes": sizes, "ms_per_step": wall_c * 1e3,
                    "trajectory_longest_chunk_ms": traj_c, "one_batch_ms_per_step": wall * 1e3,
                    "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_c - logl_support) / np.abs(logl_support)))}
         headline_wall = wall_c
@@ -488,7 +489,7 @@ def main():
         "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[15,40] M; "
                                "two-pass DOP853 rtol=atol=1e-10; (2,2) on 2^20 samples at 1/4096 s; FD likelihood (support-length transforms)",
                    "global_batch": args.n, "seq_len": pipe.N, "parallelism": "none (1 GPU)", "group_lanes": group,
-                   "block": block, "grid_groups": grid, "chunks": args.chunks},
+                   "block": block, "grid_groups": grid, "chunk_fractions": args.chunk_fractions},
         "autotune": None if tuned is None else {k: tuned[k] for k in ("group", "block", "grid_groups", "time_s", "table")},
         "stages_ms": stages, "one_batch_ms_per_step": wall * 1e3, "chunked": chunked, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
         "attempted_steps_total": int(total_attempts), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},

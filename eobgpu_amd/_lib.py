@@ -26,6 +26,7 @@ _SIGS = {
     "eobhip_last_error": [],
     "eobhip_version": [],
     "eobhip_dop853_tableau": [c_vp],
+    "eobhip_spin": [ctypes.c_longlong, c_vp],
     "eobhip_evaluate_Ham_align_AD": [c_vp] * 9 + [c_double] * 4 + [c_int, c_vp],
     "eobhip_grad_Ham_align_AD": [c_vp, c_vp, c_vp, c_int, c_vp],
     "eobhip_hessian_Ham_align_AD": [c_vp, c_vp, c_vp, c_int, c_vp],

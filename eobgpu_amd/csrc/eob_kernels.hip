@@ -548,10 +548,23 @@ int run_plain(const IntegParams& P, PlainState S, int* act0, int* act1, int* cnt
     return EOBHIP_OK;
 }
 
+// One wavefront that waits `cycles` shader clocks (bounded: it always ends).
+__global__ void k_spin(long long cycles) {
+    const long long t0 = (long long)__builtin_amdgcn_s_memtime();
+    while ((long long)__builtin_amdgcn_s_memtime() - t0 < cycles) {
+    }
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
 extern "C" {
+
+int eobhip_spin(long long cycles, void* stream) {
+    EOBHIP_REQUIRE(cycles >= 0 && cycles <= 4000000000LL, "cycles must be in [0, 4e9]");
+    hipLaunchKernelGGL(k_spin, dim3(1), dim3(64), 0, as_stream(stream), cycles);
+    return check_launch("spin");
+}
 
 const char* eobhip_last_error(void) { return g_err; }
 int eobhip_version(void) { return 1; }

@@ -105,7 +105,10 @@ class EOBBatch:
 
     # ------------------------------------------------------------------ setup
     def initial_conditions(self, r0, max_iter=1000, err=1e-12):
-        r0 = torch.as_tensor(np.broadcast_to(np.asarray(r0, dtype=np.float64), (self.n,)).copy(), device=self.device)
+        if isinstance(r0, torch.Tensor):   # a device tensor is used as is (no host round trip)
+            r0 = r0.to(device=self.device, dtype=torch.float64).reshape(self.n)
+        else:
+            r0 = torch.as_tensor(np.broadcast_to(np.asarray(r0, dtype=np.float64), (self.n,)).copy(), device=self.device)
         pphi0 = torch.empty(self.n, dtype=torch.float64, device=self.device)
         pr0 = torch.empty_like(pphi0)
         om0 = torch.empty_like(pphi0)
@@ -203,6 +206,15 @@ def fd_likelihood(htilde, dtilde, df, f0=200.0, partial=None, out=None):
     return out
 
 
+def _device_vector(x, n, dev):
+    """x (scalar, array or device tensor) as a float64 device vector of n; a device
+    tensor is used as is, so that no host copy (which waits for the stream) is made."""
+    if isinstance(x, torch.Tensor) and x.device.type == "cuda":
+        x = x.to(dtype=torch.float64).reshape(-1)
+        return x.expand(n).contiguous() if x.numel() == 1 else x.reshape(n).contiguous()
+    return torch.as_tensor(np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)).copy(), device=dev)
+
+
 class SupportLikelihood:
     """log L_b = -1/2 4 df sum_k |d~_k - rfft(h_b)_k|^2 / S(f_k) for many templates,
     computed from each template's support instead of its N-point transform.
@@ -258,22 +270,36 @@ class SupportLikelihood:
         return w
 
     def __call__(self, batch, total_mass_msun, dt, amp=None, out=None, terms=None):
+        return self.complete(self.begin(batch, total_mass_msun, dt, amp), out=out, terms=terms)
+
+    def begin(self, batch, total_mass_msun, dt, amp=None):
+        """First half of a call: support sizes on the device and their copy to the
+        host, queued on the current stream.  ``complete`` waits for that copy only,
+        so a caller can queue the first halves of several batches (on their own
+        streams) before it waits for any of them."""
         n, dev = batch.n, self.device
-        st = current_stream()
-        M = torch.as_tensor(np.broadcast_to(np.asarray(total_mass_msun, dtype=np.float64), (n,)).copy(), device=dev)
+        M = _device_vector(total_mass_msun, n, dev)
         dT_M = (dt / (M * MTSUN_SI)).contiguous()
-        if amp is None:
-            amp_t = torch.ones(n, dtype=torch.float64, device=dev)
-        else:
-            amp_t = torch.as_tensor(np.broadcast_to(np.asarray(amp, dtype=np.float64), (n,)).copy(), device=dev)
+        amp_t = torch.ones(n, dtype=torch.float64, device=dev) if amp is None else _device_vector(amp, n, dev)
         w = self._buffers(n, 0, 0, 0, 0)
         nsamp, mlog2 = w["nsamp"][:n], w["mlog2"][:n]
         call("eobhip_lik_sizes", batch.dense.data_ptr(), batch.num_steps.data_ptr(), dT_M.data_ptr(), n, batch.max_step, self.N,
-             self.m_min, nsamp.data_ptr(), mlog2.data_ptr(), st)
+             self.m_min, nsamp.data_ptr(), mlog2.data_ptr(), current_stream())
         ml_host = w["mlog2_host"][:n]
         ml_host.copy_(mlog2, non_blocking=True)
-        torch.cuda.current_stream(dev).synchronize()
-        ml = ml_host.numpy().copy()
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(dev))
+        return {"batch": batch, "n": n, "dT_M": dT_M, "amp": amp_t, "nsamp": nsamp, "mlog2": mlog2, "ml_host": ml_host,
+                "ready": ready, "stream": torch.cuda.current_stream(dev)}
+
+    def complete(self, h, out=None, terms=None):
+        """Second half: waits for the sizes, lays the templates out by transform
+        size, and queues the resample, the M-point transforms and the sums on the
+        stream ``begin`` ran on.  Returns log L [n]."""
+        batch, n, dev, st = h["batch"], h["n"], self.device, h["stream"]
+        nsamp, mlog2 = h["nsamp"], h["mlog2"]
+        h["ready"].synchronize()
+        ml = h["ml_host"].numpy().copy()
         order = np.argsort(ml, kind="stable")
         rowoff = np.empty(n, dtype=np.int64)
         xoff = np.empty(n, dtype=np.int64)
@@ -290,73 +316,130 @@ class SupportLikelihood:
         kmax = int(ml.max()) if n else self.m_min
         gx, gx2 = ctypes.c_int(0), ctypes.c_int(0)
         call("eobhip_lik_partial_blocks", kmax, ctypes.byref(gx), ctypes.byref(gx2))
-        w = self._buffers(n, ph, px, gx.value, gx2.value)
-        oh = w["offs_host"][:, :n]
-        oh[0].numpy()[:] = rowoff
-        oh[1].numpy()[:] = xoff
-        offs = w["offs"][:, :n]
-        offs.copy_(oh, non_blocking=True)
-        hbuf, X, cross, hh = w["hbuf"], w["X"], w["cross"], w["hh"]
-        call("eobhip_lik_resample", batch.dense.data_ptr(), batch.coef.data_ptr(), batch.num_steps.data_ptr(), amp_t.data_ptr(),
-             dT_M.data_ptr(), nsamp.data_ptr(), mlog2.data_ptr(), offs[0].data_ptr(), n, batch.max_step, kmax, self.g.data_ptr(),
-             hbuf.data_ptr(), cross.data_ptr(), st)
-        for Mk, h0, x0, cnt in buckets:
-            call("eobhip_rfft_batched", hbuf.data_ptr() + 8 * h0, X.data_ptr() + 16 * x0, int(cnt), int(Mk), st)
-        if out is None:
-            out = torch.empty(n, dtype=torch.float64, device=dev)
-        call("eobhip_lik_finish", X.data_ptr(), offs[1].data_ptr(), mlog2.data_ptr(), n, kmax, self.m_min, self.bank.data_ptr(),
-             cross.data_ptr(), hh.data_ptr(), self.dd.data_ptr(), self.df, out.data_ptr(), None if terms is None else terms.data_ptr(), st)
+        with torch.cuda.stream(st):
+            sp = current_stream()
+            w = self._buffers(n, ph, px, gx.value, gx2.value)
+            oh = w["offs_host"][:, :n]
+            oh[0].numpy()[:] = rowoff
+            oh[1].numpy()[:] = xoff
+            offs = w["offs"][:, :n]
+            offs.copy_(oh, non_blocking=True)
+            hbuf, X, cross, hh = w["hbuf"], w["X"], w["cross"], w["hh"]
+            call("eobhip_lik_resample", batch.dense.data_ptr(), batch.coef.data_ptr(), batch.num_steps.data_ptr(),
+                 h["amp"].data_ptr(), h["dT_M"].data_ptr(), nsamp.data_ptr(), mlog2.data_ptr(), offs[0].data_ptr(), n,
+                 batch.max_step, kmax, self.g.data_ptr(), hbuf.data_ptr(), cross.data_ptr(), sp)
+            for Mk, h0, x0, cnt in buckets:
+                call("eobhip_rfft_batched", hbuf.data_ptr() + 8 * h0, X.data_ptr() + 16 * x0, int(cnt), int(Mk), sp)
+            if out is None:
+                out = torch.empty(n, dtype=torch.float64, device=dev)
+            call("eobhip_lik_finish", X.data_ptr(), offs[1].data_ptr(), mlog2.data_ptr(), n, kmax, self.m_min, self.bank.data_ptr(),
+                 cross.data_ptr(), hh.data_ptr(), self.dd.data_ptr(), self.df, out.data_ptr(),
+                 None if terms is None else terms.data_ptr(), sp)
         self.nsamp, self.mlog2 = nsamp, mlog2
-        self._tmp = (dT_M, amp_t)
+        self._tmp = (h["dT_M"], h["amp"])
         return out
 
 
-def expected_length(r0, m1, m2):
-    """A monotone proxy of a binary's number of integrator steps: orbits to merger
-    grow as r0^(5/2) / nu (leading post-Newtonian order), and the adaptive step is
-    a roughly fixed fraction of an orbit.  Used only to group binaries of similar
-    length; it changes no result."""
+def _runs_beside(a, b, cycles):
+    """True if a short kernel on stream b finishes while a spin of `cycles` runs on a."""
+    start = torch.cuda.Event(enable_timing=True)
+    end_a = torch.cuda.Event(enable_timing=True)
+    end_b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(a)
+    call("eobhip_spin", int(cycles), a.cuda_stream)
+    end_a.record(a)
+    b.wait_event(start)
+    call("eobhip_spin", 0, b.cuda_stream)
+    end_b.record(b)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end_b) < 0.5 * start.elapsed_time(end_a)
+
+
+def concurrent_streams(k, device="cuda", cycles=2_000_000, candidates=32):
+    """k streams (default priority, from torch's pool) whose kernels run
+    concurrently with each other's.  A process gets a few hardware queues
+    (GPU_MAX_HW_QUEUES); further streams share them, and a kernel queued behind
+    a long one in a shared queue waits for it -- which would serialise chunks
+    meant to overlap.  Each candidate is tested against the streams already
+    taken, in both directions, with a bounded spin kernel.  Returns fewer than
+    k streams if the candidates run out."""
+    chosen = []
+    for _ in range(candidates):
+        if len(chosen) == k:
+            break
+        s = torch.cuda.Stream(device=device)
+        call("eobhip_spin", 0, s.cuda_stream)   # first use of the stream
+        if all(_runs_beside(c, s, cycles) and _runs_beside(s, c, cycles) for c in chosen):
+            chosen.append(s)
+    torch.cuda.synchronize()
+    return chosen
+
+
+def expected_length(r0, m1, m2, chi1=0.0, chi2=0.0):
+    """Expected attempted steps of a binary's two-pass integration, up to a
+    factor: r0^1.165 nu^-0.292 exp(0.150 chi_eff), a least-squares fit of
+    log(attempts) to log r0, log nu and chi_eff over the 1024-binary uneven
+    batch (rank correlation 0.98; orbits to merger, r0^(5/2)/nu, rank 0.96,
+    puts more of the longest binaries among shorter ones).  Used only to group
+    binaries of similar length; it changes no result."""
     m1 = np.asarray(m1, dtype=np.float64)
     m2 = np.asarray(m2, dtype=np.float64)
     nu = m1 * m2 / (m1 + m2) ** 2
-    return np.asarray(r0, dtype=np.float64) ** 2.5 / nu
+    chi_eff = (m1 * np.asarray(chi1, dtype=np.float64) + m2 * np.asarray(chi2, dtype=np.float64)) / (m1 + m2)
+    return np.exp(1.165 * np.log(np.asarray(r0, dtype=np.float64)) - 0.292 * np.log(nu) + 0.150 * chi_eff)
 
 
 class ChunkedPass:
     """One pass over n binaries -- tables, initial conditions, the two-pass
     trajectory, the (2,2) knots, the splines and the support-length likelihood
-    (``SupportLikelihood``) -- with the batch split into ``chunks`` groups of
-    binaries of similar expected length, each with its own buffers and stream.
+    (``SupportLikelihood``) -- with the batch split into groups of binaries of
+    similar expected length (``fractions`` of the batch, longest first), each
+    with its own buffers and stream.  Three chunks by default: with the
+    caller's stream they use four hardware queues, the number a process gets
+    (GPU_MAX_HW_QUEUES); streams beyond that share a queue, and work queued
+    behind a long integrator in a shared queue waits for it.
 
     Every chunk's persistent integrator starts at once, so together they occupy
     the SIMDs as one launch would.  The chunks of shorter binaries finish first
     and run their knots, splines, transforms and sums while the longer chunks
     are still integrating: that downstream work fills the integrator's tail,
-    in which only the longest binaries' wavefronts still run.  Each binary's
+    in which only the longest binaries' wavefronts still run.  The chunk of
+    the longest binaries is the smallest, since its downstream work is the
+    part left after the last trajectory ends.  Each binary's
     numbers are those of the one-batch pass (the kernels treat binaries
     independently; the transforms run on other batch sizes, so log L may
     differ in the last bits).
     """
 
-    def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, chunks=4, rtol=1e-10,
-                 device="cuda", order=None):
+    def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, fractions=(0.1, 0.3, 0.6),
+                 rtol=1e-10, device="cuda", order=None):
         m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
         n = m1.size
-        if chunks < 1 or chunks > n:
-            raise ValueError("chunks must be in [1, n]")
+        fr = np.asarray(fractions, dtype=np.float64)
+        if fr.ndim != 1 or fr.size < 1 or fr.size > n or np.any(fr <= 0):
+            raise ValueError("fractions: 1..n positive shares of the batch, longest binaries first")
         if order is None:
-            order = np.argsort(-expected_length(r0, m1, m2), kind="stable")   # longest first
+            order = np.argsort(-expected_length(r0, m1, m2, chi1, chi2), kind="stable")   # longest first
         self.n, self.lik, self.dt, self.rtol, self.device = n, lik, float(dt), float(rtol), device
         M = np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))
         A = np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (n,))
-        self.groups = [np.asarray(g) for g in np.array_split(np.asarray(order), chunks)]
-        self.streams = [torch.cuda.Stream(device=device) for _ in self.groups]
+        # chunk sizes from the shares, each at least one binary
+        cuts = np.round(np.cumsum(fr / fr.sum()) * n).astype(np.int64)[:-1]
+        cuts = np.clip(np.maximum(cuts, np.arange(1, fr.size)), None, n - (fr.size - 1) + np.arange(fr.size - 1))
+        streams = concurrent_streams(fr.size, device=device)
+        if len(streams) < fr.size:
+            raise RuntimeError(f"found {len(streams)} streams that run concurrently, {fr.size} chunks need as many")
+        self.groups = [np.asarray(g) for g in np.split(np.asarray(order), cuts)]
+        self.streams = streams
         self.batches, self.r0, self.M, self.amp, self.idx, self.outs = [], [], [], [], [], []
         for g in self.groups:
             self.batches.append(EOBBatch(make_additional_args(m1[g], m2[g], chi1[g], chi2[g]), max_step=max_step, device=device))
-            self.r0.append(r0[g].copy())
-            self.M.append(M[g].copy())
-            self.amp.append(A[g].copy())
+            # per-chunk inputs live on the device: a host array would be copied
+            # (and its stream waited for) inside the pass
+            self.r0.append(torch.as_tensor(r0[g].copy(), device=device))
+            self.M.append(torch.as_tensor(M[g].copy(), device=device))
+            self.amp.append(torch.as_tensor(A[g].copy(), device=device))
             self.idx.append(torch.as_tensor(g, dtype=torch.int64, device=device))
             self.outs.append(torch.empty(g.size, dtype=torch.float64, device=device))
         self.out = torch.empty(n, dtype=torch.float64, device=device)
@@ -376,12 +459,16 @@ class ChunkedPass:
                 b.integrate(rtol=self.rtol, atol=self.rtol, two_pass=True, group=group, block=block, grid_groups=grid_groups)
                 if k == 0 and events is not None:
                     events["traj_end_longest"].record(st)
+        pending = []
         for k in reversed(range(len(self.batches))):   # shortest first: they finish first
             b, st = self.batches[k], self.streams[k]
             with torch.cuda.stream(st):
                 b.hlm22_knots(out=getattr(b, "knots", None))
                 b.build_splines()
-                self.lik(b, self.M[k], self.dt, amp=self.amp[k], out=self.outs[k])
+                pending.append((k, self.lik.begin(b, self.M[k], self.dt, amp=self.amp[k])))
+        for k, h in pending:
+            self.lik.complete(h, out=self.outs[k])
+            with torch.cuda.stream(self.streams[k]):
                 self.out.index_copy_(0, self.idx[k], self.outs[k])
         for st in self.streams:
             cur.wait_stream(st)

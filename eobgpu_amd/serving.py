@@ -133,7 +133,11 @@ class Server:
         # lanes per binary of the plain policy: the same as the persistent one by
         # default, so that both policies give bit-identical results
         self.plain_group = group if plain_group is None else plain_group
-        self.streams = [torch.cuda.Stream(device=device) for _ in range(max(1, int(streams)))]
+        # streams whose kernels run concurrently (a stream that shares a hardware
+        # queue with another would hold its requests behind the other's)
+        self.streams = P.concurrent_streams(max(1, int(streams)), device=device)
+        if not self.streams:
+            raise RuntimeError("no stream available")
         self._stream_load = [0] * len(self.streams)
 
     def warm(self):

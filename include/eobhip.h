@@ -37,6 +37,13 @@ int eobhip_version(void);
  * the constants against a published table. */
 int eobhip_dop853_tableau(double* out_host);
 
+/* One wavefront that waits `cycles` shader clocks on `stream` (at most 4e9).
+ * No reference counterpart: eobgpu_amd.pipeline.concurrent_streams launches it
+ * to find streams whose kernels run concurrently (streams beyond the process's
+ * hardware queues share one, and work behind a long kernel in a shared queue
+ * waits for it). */
+int eobhip_spin(long long cycles, void* stream);
+
 /* ---------------------------------------------------------------- eob.pyx */
 /* evaluate_Ham_align_AD_wrap, include/EOB.hh:11; src/eob.pyx:92-102 */
 int eobhip_evaluate_Ham_align_AD(double* out, const double* r, const double* phi, const double* pr, const double* pphi,

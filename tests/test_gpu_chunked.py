@@ -32,7 +32,7 @@ def test_chunked_pass_matches_one_batch(dev):
     b.build_splines()
     ref = lik(b, M, DT, amp=amp).cpu().numpy()
 
-    cp = P.ChunkedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], lik, M, DT, amp=amp, max_step=2048, chunks=3)
+    cp = P.ChunkedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], lik, M, DT, amp=amp, max_step=2048, fractions=(0.2, 0.3, 0.5))
     for _ in range(2):   # a second pass reuses the chunks' buffers
         got = cp.run().cpu().numpy()
         np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
@@ -45,5 +45,5 @@ def test_chunked_pass_matches_one_batch(dev):
         L = int(ns.max())
         np.testing.assert_array_equal(cb.dense[:, :L].cpu().numpy(), b.dense[torch.as_tensor(g, device=dev)][:, :L].cpu().numpy())
     # the longest expected binaries went to the first chunk
-    key = P.expected_length(p["r0"], p["m1"], p["m2"])
+    key = P.expected_length(p["r0"], p["m1"], p["m2"], p["chi1"], p["chi2"])
     assert key[cp.groups[0]].min() >= key[cp.groups[-1]].max()
