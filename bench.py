@@ -35,6 +35,10 @@ sys.path.insert(0, ROOT)
 # error norm + controller), counted by eobgpu_amd/csrc/flop_count.cpp on the
 # library's own Hamiltonian and flux code.
 STEP_FLOPS = 48395
+# HBM traffic of one integrator launch on the headline batch (rocprofv3 FETCH_SIZE
+# and WRITE_SIZE, each scaled by the counter's reading for a 1 GiB device copy;
+# profiles/pipeline_uneven1024.md): 26.23 MB read + 27.75 MB written.
+INTEGRATOR_TRAFFIC_BYTES = (26.23 + 27.75) * 1e6
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
@@ -58,7 +62,7 @@ def parse():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--chunk-fractions", default="0.1,0.3,0.6",
+    ap.add_argument("--chunk-fractions", default="0.1,0.2,0.7",
                     help="the headline pass splits the batch into chunks of these shares (longest binaries first) on their "
                          "own streams (eobgpu_amd.pipeline.ChunkedPass); '1' times the one-batch pass")
     ap.add_argument("--copy-probe", action="store_true",
@@ -463,7 +467,7 @@ def main():
     if dom == "trajectory":
         achieved = total_attempts * STEP_FLOPS / (stages["trajectory"] * 1e-3) / 1e12
         roof = {"bound": "fp64-vector", "kernel": "k_integrate_persistent", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": INTEGRATOR_TRAFFIC_BYTES,
                 "algorithmic": f"{STEP_FLOPS} FP64 ops per attempted step x {int(total_attempts)} attempted steps"}
     else:
         # support path: h on M_b samples written and read by the FFT, the M_b/2+1 bins read back

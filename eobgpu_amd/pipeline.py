@@ -412,7 +412,7 @@ class ChunkedPass:
     differ in the last bits).
     """
 
-    def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, fractions=(0.1, 0.3, 0.6),
+    def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, fractions=(0.1, 0.2, 0.7),
                  rtol=1e-10, device="cuda", order=None):
         m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
         n = m1.size
