@@ -245,12 +245,12 @@ __device__ __forceinline__ double group_flux(const ModeCoef* __restrict__ mc, do
     double v, cb_in;
     if constexpr (G >= 4) {
         // the two cube roots in one call, on two lanes
-        const double cr = cbrt((gl & 1) ? vPhiIn : omega);
+        const double cr = cbrt_pos((gl & 1) ? vPhiIn : omega);
         v = gread<G>(cr, 0);
         cb_in = gread<G>(cr, 1);
     } else {
-        v = cbrt(omega);
-        cb_in = cbrt(vPhiIn);
+        v = cbrt_pos(omega);
+        cb_in = cbrt_pos(vPhiIn);
     }
     const double logv = log_pos(v);
     const double vPhi = r * cb_in * omega;

@@ -17,3 +17,6 @@ extern "C" void probe_qrcp(const double* b, double* y, int n) {
 extern "C" void probe_qsqrt(const double* x, double* y, int n) {
     for (int i = 0; i < n; ++i) y[i] = eob::qsqrt(x[i]);
 }
+extern "C" void probe_cbrt_pos(const double* x, double* y, int n) {
+    for (int i = 0; i < n; ++i) y[i] = eob::cbrt_pos(x[i]);
+}

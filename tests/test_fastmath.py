@@ -6,7 +6,8 @@ whole exponent range; NaN outside the positive finite numbers (the header is
 compiled with contraction off, which rounds as the device build does).
 qdiv / qrcp / qsqrt, started from single-precision estimates, against IEEE
 division and square root (Python's, correctly rounded): within 1 ulp, and
-equal in all but a handful of cases."""
+equal in all but a handful of cases.  cbrt_pos against cube roots from
+decimal: within 1 ulp."""
 import ctypes
 import math
 import os
@@ -29,7 +30,7 @@ def probe(tmp_path_factory):
     subprocess.run([cxx, "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", out,
                     os.path.join(HERE, "fastmath_probe.cpp")], check=True)
     lib = ctypes.CDLL(out)
-    for name in ("probe_log_pos", "probe_qrcp", "probe_qsqrt"):
+    for name in ("probe_log_pos", "probe_qrcp", "probe_qsqrt", "probe_cbrt_pos"):
         getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     lib.probe_qdiv.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int]
 
@@ -89,3 +90,18 @@ def test_division_and_square_root(probe):
         u = _ulps(got, ref)
         assert u.max() <= 1.0
         assert np.count_nonzero(got != ref) <= 5
+
+
+
+def test_cube_root(probe):
+    from decimal import Decimal, getcontext
+    getcontext().prec = 40
+    rng = np.random.default_rng(14)
+    x = np.concatenate([10.0 ** rng.uniform(-6, 8, 6000), rng.uniform(0.5, 8.0, 2000), np.array([1.0, 8.0, 27.0, 0.125])])
+    y = probe(x, "probe_cbrt_pos")
+    worst = 0.0
+    for xi, yi in zip(x, y):
+        ref = Decimal(float(xi)) ** (Decimal(1) / 3)
+        worst = max(worst, float(abs(Decimal(float(yi)) - ref) / Decimal(math.ulp(float(ref)))))
+    assert worst <= 1.0
+    assert np.all(np.isnan(probe(np.array([0.0, np.nan]), "probe_cbrt_pos")))
