@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 # FP64 operations per attempted DOP853 step (12 right-hand sides + stages +
 # error norm + controller), counted by eobgpu_amd/csrc/flop_count.cpp on the
 # library's own Hamiltonian and flux code.
-STEP_FLOPS = 48395
+STEP_FLOPS = 50663
 # HBM traffic of one integrator launch on the headline batch (rocprofv3 FETCH_SIZE
 # and WRITE_SIZE, each scaled by the counter's reading for a 1 GiB device copy;
 # profiles/pipeline_uneven1024.md): 26.23 MB read + 27.75 MB written.

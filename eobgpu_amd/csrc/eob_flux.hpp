@@ -28,6 +28,7 @@ struct ModeCoef {
     double f1, f3;                 // auxiliary f_lm: f1 v + f3 v^3
     double N2;                     // |prefix_lm|^2 |Y_{l-eps}^{-m}(pi/2,0)|^2
     double inv_lfac2;              // 1/(l!)^2
+    double pp[9];                  // prod_{i=1..l} (h + i^2) = sum_j pp[j] h^j (integers; zero above j = l)
     double log2m;                  // ln(2m)
     double m2;                     // m^2
     int l, m, lpe, flags;          // lpe = l + eps; flags: bit0 even (l+m), bit1 f-only (eta = 1/4, odd m)
@@ -92,6 +93,13 @@ EOB_HD void flux_setup(ModeCoef* mc, WaveCoef& wc, double m1, double m2, double 
         double lf = 1.0;
         for (int q = 2; q <= l; ++q) lf *= q;
         z.inv_lfac2 = 1.0 / (lf * lf);
+        // expand prod_{i=1..l} (h + i^2) one factor at a time (exact: the
+        // coefficients stay below 2^53)
+        for (int j = 0; j < 9; ++j) z.pp[j] = (j == 0) ? 1.0 : 0.0;
+        for (int q = 1; q <= l; ++q) {
+            for (int j = 8; j >= 1; --j) z.pp[j] = z.pp[j - 1] + (double)(q * q) * z.pp[j];
+            z.pp[0] *= (double)(q * q);
+        }
         z.log2m = log(2.0 * m);
         z.m2 = (double)(m * m);
         const double pa2 = prefixes ? prefixes[2 * i] * prefixes[2 * i] + prefixes[2 * i + 1] * prefixes[2 * i + 1] : 0.0;
@@ -310,10 +318,12 @@ EOB_HD T flux_mode(const ModeCoef& z, T v, T logv, T H, T Omega, T Heff, T pphi,
     const T x = 4.0 * kPi * hk;
     const T Em = exp(-x);
     const T hk4 = 4.0 * hk * hk;
-    T prod = hk4 + 1.0;   // every mode has l >= 2
-    prod = prod * (hk4 + 4.0);
+    // prod_{i=1..l} (hk4 + i^2) as a polynomial in hk4 (the same degree-8
+    // Horner on every lane; all terms are positive, so it loses nothing to
+    // cancellation against the reference's running product)
+    T prod = z.pp[8];
 #pragma unroll
-    for (int i = 3; i <= 8; ++i) prod = (i <= z.l) ? prod * (hk4 + (double)(i * i)) : prod;
+    for (int j = 7; j >= 0; --j) prod = prod * hk4 + z.pp[j];
     const T T2 = qdiv(x, T(1.0) - Em) * prod * z.inv_lfac2;
     const T el = kEulerGamma + z.log2m + logv;
     // Horner over v^12 .. v^1

@@ -335,12 +335,4 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     return p;
 }
 
-// serial form (one lane evaluates all pieces)
-EOB_HD HamPack ham_pack_serial(const HamCoef& c, const PieceCoef& pc, double r, double pr, double L) {
-    const double u = qrcp(r), lr = log_pos(r), su = qsqrt(u), hs = 0.5 * su * r;
-    double v[kNumPieces], d[kNumPieces];
-    for (int q = 0; q < kNumPieces; ++q) piece_eval(pc.c[q], u, r, lr, su, hs, v[q], d[q]);
-    return ham_combine(c, r, u, pr, L, v, d);
-}
-
 }  // namespace eob
