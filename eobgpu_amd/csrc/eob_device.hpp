@@ -191,7 +191,11 @@ __device__ __forceinline__ double* k1_slots() {
 template <int G>
 __device__ __forceinline__ HamPack ham_pack(const HamCoef& hc, const PieceCoef& pcs, double r, double pr, double L, int gl,
                                             const double (*cf)[3] = nullptr) {
-    const double u = qrcp(r), lr = log_pos(r), su = qsqrt(u), hs = 0.5 * su * r;
+    // sqrt(r) and 1/sqrt(r) in one call: su = sqrt(u), u = su^2, and
+    // hs = d su/du = su r / 2 = sqrt(r) / 2
+    double sr, su;
+    qsqrt_recip(r, sr, su);
+    const double u = su * su, lr = log_pos(r), hs = 0.5 * sr;
     double v[kNumPieces], d[kNumPieces];
     if constexpr (G == 64) {
         // gathered through this wavefront's LDS slots: one write, then 13

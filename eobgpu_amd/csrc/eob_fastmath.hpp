@@ -126,6 +126,28 @@ EOB_HD double qsqrt(double x) {
     return g;
 }
 
+// sqrt(x) and 1/sqrt(x) together: the square root as qsqrt, and 2h (which
+// Goldschmidt's iteration carries as 1/sqrt(x) to about 2^-44) refined by one
+// Newton step for 1/g, within 1 ulp -- two fused multiply-adds instead of a
+// separate reciprocal.
+template <class T>
+EOB_HD void qsqrt_recip(T x, T& root, T& inv) {
+    root = sqrt(x);
+    inv = 1.0 / root;
+}
+EOB_HD void qsqrt_recip(double x, double& root, double& inv) {
+    const double y = rsqrt_estimate(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    g = fma(fma(-g, g, x), h, g);
+    g = fma(fma(-g, g, x), h, g);
+    const double y2 = h + h;
+    root = g;
+    inv = fma(y2, fma(-g, y2, 1.0), y2);
+}
+
 // cube root of a positive normal x: a single-precision estimate of x^(-1/3)
 // (hardware log2 and exp2), two Newton steps for z = x^(-1/3) (z += z (1 - x
 // z^3) / 3, no division), y = x z^2 and one residual correction

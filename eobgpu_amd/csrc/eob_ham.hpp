@@ -305,16 +305,16 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     const T dQ = P2 * (d[kQ4] + P * (d[kQ6] + P * d[kQ8]));
     const T QP = P * (2.0 * v[kQ4] + P * (3.0 * v[kQ6] + 4.0 * P * v[kQ8]));   // dQ/dP
     const T X = (Bnp + 1.0) * P + W * L2 + Q + 1.0;
-    const T Heven = qsqrt(A * X);
-    const T iHe = qrcp(Heven);
+    T Heven, iHe;
+    qsqrt_recip(A * X, Heven, iHe);
     const T ihe = 0.5 * iHe;
     const T Gt = v[kG0] + P * (v[kG1] + P * (v[kG2] + P * (v[kG3] + P * c.g40))) + v[kGS];
     const T dGt = d[kG0] + P * (d[kG1] + P * (d[kG2] + P * d[kG3])) + d[kGS];
     const T GtP = v[kG1] + P * (2.0 * v[kG2] + P * (3.0 * v[kG3] + 4.0 * P * c.g40));
     const T U3i = u3 * iDn, dU3i = 3.0 * u2 * iDn + u3 * diDn;
     const T Heff = Heven + L * U3i * Gt;
-    const T s = qsqrt(1.0 + 2.0 * c.nu * (Heff - 1.0));
-    const T is = qrcp(s);
+    T s, is;
+    qsqrt_recip(1.0 + 2.0 * c.nu * (Heff - 1.0), s, is);
     const T kH = c.M * is;   // dH/dHeff
     const T dHe_du = (dA * X + A * (dBnp * P + dW * L2 + dQ)) * ihe + L * (dU3i * Gt + U3i * dGt);
     const T dHe_dP = A * (Bnp + 1.0 + QP) * ihe + L * U3i * GtP;
@@ -327,10 +327,11 @@ EOB_HD HamPackT<T> ham_combine(const HamCoef& c, T r, T u, T pr, T L, const T* v
     if constexpr (Br::kLaneP0) {
         p.omc = br.omc(p.dHdL);
     } else {
-        const T He0 = qsqrt(A * (W * L2 + 1.0));
+        T He0, iHe0, s0, is0;
+        qsqrt_recip(A * (W * L2 + 1.0), He0, iHe0);
         const T Gt0 = v[kG0] + v[kGS];
-        const T s0 = qsqrt(1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0));
-        p.omc = (c.M * qrcp(s0)) * (A * L * W * qrcp(He0) + U3i * Gt0);
+        qsqrt_recip(1.0 + 2.0 * c.nu * (He0 + L * U3i * Gt0 - 1.0), s0, is0);
+        p.omc = (c.M * is0) * (A * L * W * iHe0 + U3i * Gt0);
     }
     return p;
 }

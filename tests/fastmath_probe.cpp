@@ -20,3 +20,6 @@ extern "C" void probe_qsqrt(const double* x, double* y, int n) {
 extern "C" void probe_cbrt_pos(const double* x, double* y, int n) {
     for (int i = 0; i < n; ++i) y[i] = eob::cbrt_pos(x[i]);
 }
+extern "C" void probe_qsqrt_recip(const double* x, double* r, double* inv, int n) {
+    for (int i = 0; i < n; ++i) eob::qsqrt_recip(x[i], r[i], inv[i]);
+}

@@ -33,10 +33,15 @@ def probe(tmp_path_factory):
     for name in ("probe_log_pos", "probe_qrcp", "probe_qsqrt", "probe_cbrt_pos"):
         getattr(lib, name).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     lib.probe_qdiv.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int]
+    lib.probe_qsqrt_recip.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int]
 
     def run(x, fn="probe_log_pos", b=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.empty_like(x)
+        if fn == "probe_qsqrt_recip":
+            inv = np.empty_like(x)
+            lib.probe_qsqrt_recip(x.ctypes.data, y.ctypes.data, inv.ctypes.data, len(x))
+            return y, inv
         if b is None:
             getattr(lib, fn)(x.ctypes.data, y.ctypes.data, len(x))
         else:
@@ -85,6 +90,9 @@ def test_division_and_square_root(probe):
     rng = np.random.default_rng(12)
     a = rng.choice([-1.0, 1.0], 50000) * 10.0 ** rng.uniform(-8, 8, 50000)
     b = 10.0 ** rng.uniform(-8, 8, 50000)
+    root, inv = probe(b, "probe_qsqrt_recip")
+    np.testing.assert_array_equal(root, probe(b, "probe_qsqrt"))
+    assert np.max(_ulps(inv, (1.0 / np.sqrt(np.float128(b))).astype(np.float64))) <= 1.0   # 80-bit reference
     for got, ref in ((probe(a, "probe_qdiv", b), a / b), (probe(b, "probe_qrcp"), 1.0 / b),
                      (probe(b, "probe_qsqrt"), np.sqrt(b))):
         u = _ulps(got, ref)
