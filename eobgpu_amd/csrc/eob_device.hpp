@@ -126,6 +126,15 @@ __device__ __forceinline__ double wave_sum(double v) {
     return read_lane(v, 63);
 }
 
+// Sum over lanes 0..3 of a 64-lane group whose other lanes hold 0 (the four
+// state components): two swaps inside the quad, then lane 0 -- the same
+// additions, in the same order, as wave_sum makes for these values.
+__device__ __forceinline__ double quad0_sum(double v) {
+    v += dpp_perm<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dpp_perm<0x4E>(v);    // quad_perm [2,3,0,1]
+    return read_lane(v, 0);
+}
+
 template <int G>
 __device__ __forceinline__ double gsum(double v) {
     if constexpr (G == 64) {

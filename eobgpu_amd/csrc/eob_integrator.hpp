@@ -290,8 +290,13 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
             e5 += t5s * t5s;
         }
     }
-    e3 = gsum<G>(e3);
-    e5 = gsum<G>(e5);
+    if constexpr (G == 64) {
+        e3 = quad0_sum(e3);
+        e5 = quad0_sum(e5);
+    } else {
+        e3 = gsum<G>(e3);
+        e5 = gsum<G>(e5);
+    }
     double denom = e5 + 0.01 * e3;
     denom = (denom < 0.0) ? 1.0 : denom;
     const double err = fabs(h) * e5 * sqrt(1.0 / (4.0 * denom));
