@@ -62,9 +62,11 @@ def parse():
     ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--max-step", type=int, default=4096)
     ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--pass", dest="pass_kind", choices=("streamed", "chunked", "one-batch"), default="streamed",
+                    help="the headline pass: streamed (eobgpu_amd.pipeline.StreamedPass: one integrator launch, downstream "
+                         "work per group of finished binaries), chunked (ChunkedPass) or the one-batch pass")
     ap.add_argument("--chunk-fractions", default="0.1,0.2,0.7",
-                    help="the headline pass splits the batch into chunks of these shares (longest binaries first) on their "
-                         "own streams (eobgpu_amd.pipeline.ChunkedPass); '1' times the one-batch pass")
+                    help="ChunkedPass shares (longest binaries first), timed beside the headline pass; '1' skips it")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
     ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
@@ -180,6 +182,31 @@ def time_chunked(pipe, fractions, steps, warmup, group=64, block=64):
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
     return wall, cp.out.cpu().numpy(), t_traj / steps, [len(g) for g in cp.groups]
+
+
+def time_streamed(pipe, steps, warmup, group=64, block=64, grid_groups=0):
+    """The same pass as Pipeline.run (support-length likelihood) as a StreamedPass:
+    one integrator launch that flags each finished binary, knots, splines and
+    likelihood queued per group of finished binaries; returns (wall per pass,
+    log L, trajectory time, downstream groups of the last pass)."""
+    torch, P = pipe.torch, pipe.P
+    p = pipe.params
+    sp = P.StreamedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], pipe.lik, pipe.M, pipe.dt, amp=pipe.amp,
+                        max_step=pipe.b.max_step, rtol=pipe.rtol)
+    for _ in range(warmup):
+        sp.run(group=group, block=block, grid_groups=grid_groups)
+    torch.cuda.synchronize()
+    t_traj = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev = {"start": torch.cuda.Event(enable_timing=True), "traj_end_longest": torch.cuda.Event(enable_timing=True)}
+        ev["start"].record()
+        sp.run(group=group, block=block, grid_groups=grid_groups, events=ev)
+        torch.cuda.synchronize()
+        t_traj += ev["start"].elapsed_time(ev["traj_end_longest"])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    return wall, sp.out.cpu().numpy(), t_traj / steps, len(sp.groups)
 
 
 def cpu_baseline(pipe, nb):
@@ -435,16 +462,24 @@ def main():
                    "max_rel_diff_logl": float(np.max(np.abs(logl_support - logl_direct) / np.abs(logl_direct)))}
     print(f"[bench] full-length likelihood path: {json.dumps(full_length)}", file=sys.stderr, flush=True)
 
-    chunked = None
+    chunked = streamed = None
     headline_wall = wall
     fractions = [float(x) for x in args.chunk_fractions.split(",") if x.strip()]
-    if len(fractions) > 1:
+    if len(fractions) > 1 or args.pass_kind == "chunked":
         wall_c, logl_c, traj_c, sizes = time_chunked(pipe, fractions, args.steps, args.warmup, group=group)
         chunked = {"fractions": fractions, "chunk_sizes": sizes, "ms_per_step": wall_c * 1e3,
                    "trajectory_longest_chunk_ms": traj_c, "one_batch_ms_per_step": wall * 1e3,
                    "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_c - logl_support) / np.abs(logl_support)))}
-        headline_wall = wall_c
+        if args.pass_kind == "chunked":
+            headline_wall = wall_c
         print(f"[bench] chunked pass: {json.dumps(chunked)}", file=sys.stderr, flush=True)
+    if args.pass_kind == "streamed":
+        wall_s, logl_s, traj_s, ngroups = time_streamed(pipe, args.steps, args.warmup, group=group, block=block, grid_groups=grid)
+        streamed = {"ms_per_step": wall_s * 1e3, "trajectory_ms": traj_s, "downstream_groups": ngroups,
+                    "one_batch_ms_per_step": wall * 1e3,
+                    "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_s - logl_support) / np.abs(logl_support)))}
+        headline_wall = wall_s
+        print(f"[bench] streamed pass: {json.dumps(streamed)}", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:
@@ -493,9 +528,9 @@ def main():
         "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[15,40] M; "
                                "two-pass DOP853 rtol=atol=1e-10; (2,2) on 2^20 samples at 1/4096 s; FD likelihood (support-length transforms)",
                    "global_batch": args.n, "seq_len": pipe.N, "parallelism": "none (1 GPU)", "group_lanes": group,
-                   "block": block, "grid_groups": grid, "chunk_fractions": args.chunk_fractions},
+                   "block": block, "grid_groups": grid, "pass": args.pass_kind},
         "autotune": None if tuned is None else {k: tuned[k] for k in ("group", "block", "grid_groups", "time_s", "table")},
-        "stages_ms": stages, "one_batch_ms_per_step": wall * 1e3, "chunked": chunked, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
+        "stages_ms": stages, "one_batch_ms_per_step": wall * 1e3, "streamed": streamed, "chunked": chunked, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
         "attempted_steps_total": int(total_attempts), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
         "logl_finite": bool(np.all(np.isfinite(logl))),
         "full_length_likelihood": full_length, "plain_batch": plain, "served": served, "roofline": roof, "cpu_baseline": cpu,

@@ -50,6 +50,9 @@ _SIGS = {
     "eobhip_integrate": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
                          c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                          c_ll_p, c_vp],
+    "eobhip_integrate_signal": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
+                                c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                c_vp, c_vp],
     "eobhip_hlm22_knots": [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp],
 }
 _RESTYPES = {

@@ -50,9 +50,12 @@ def candidates(n, groups=DEFAULT_GROUPS, blocks=DEFAULT_BLOCKS, cu=256):
                 continue
             per_block = bl // g
             # one wavefront per SIMD fits (4 per CU); launch all binaries as groups,
-            # or just enough groups to fill the resident slots and refill from the queue
+            # just enough groups to fill the resident slots and refill from the
+            # queue, three quarters of them (three wavefronts per CU run each a
+            # little faster than four), or half
             resident = cu * 4 * (64 // g)
-            for gg in sorted({n, min(n, resident), min(n, max(per_block, min(n, resident) // 2))}):
+            for gg in sorted({n, min(n, resident), min(n, max(per_block, resident * 3 // 4)),
+                              min(n, max(per_block, min(n, resident) // 2))}):
                 if gg % per_block and gg != n:
                     continue
                 out.append((g, bl, gg))

@@ -45,6 +45,7 @@ struct IntegParams {
     const double* x0;         // optional restart: start time per binary (with step0)
     const int* step0;         // optional restart: row the initial state occupies
     int use_stop;             // StoppingCriterion on (the reference driver without one uses stop code 0)
+    int* done;                // optional [nbin], host memory mapped for the device: 1 once the binary's outputs are written
 };
 
 // err^kAlpha for the controller's err^-kAlpha (src/DOPR853.cu:675-762 calls
@@ -206,6 +207,15 @@ __device__ __forceinline__ void finish_binary(const IntegParams& P, GState<G>& s
         P.stop_info[s.b] = s.code;
         P.pass1_steps[s.b] = P.two_pass ? s.sn1 : s.step_num;
         P.attempts[s.b] = s.att;
+    }
+    if (P.done) {
+        // the rows and the four outputs above reach memory before the flag: a
+        // system-scope release fence by the whole wavefront (it writes the
+        // cache's dirty lines back and, unlike a full fence, leaves the cache's
+        // contents valid for the other wavefronts), then lane 0's system-scope
+        // store, which goes through to host memory
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (gl == 0) __hip_atomic_store(P.done + s.b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
     s.pass = 0;
 }

@@ -382,14 +382,27 @@ __device__ __forceinline__ void stage_tables(const IntegParams& P, int b, BinDat
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// LDS per wavefront of a 64-lane-group launch: its binary's tables.  The
+// launch sizes the dynamic LDS by the wavefronts of a block, so a one-wavefront
+// block holds one copy (about 11.5 KB) and leaves the rest of the CU's LDS to
+// other kernels.
+constexpr size_t kStageBytesPerWave = sizeof(BinData) + kNumModes * sizeof(ModeCoef);
+static_assert(sizeof(BinData) % 16 == 0 && kStageBytesPerWave % 16 == 0, "16-byte aligned tables");
+
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) {
     // 64-lane groups own a wavefront: each keeps its binary's tables in LDS
     constexpr bool kStage = (G == 64);
-    __shared__ BinData lds_bd[kStage ? kBlock / 64 : 1];
-    __shared__ ModeCoef lds_mc[kStage ? kBlock / 64 : 1][kNumModes];
+    extern __shared__ __align__(16) unsigned char stage_lds[];
     const int wave = threadIdx.x >> 6;
     const int gl = threadIdx.x % G;
+    BinData* lbd = nullptr;
+    ModeCoef* lmc = nullptr;
+    if constexpr (kStage) {
+        unsigned char* base = stage_lds + (size_t)wave * kStageBytesPerWave;
+        lbd = reinterpret_cast<BinData*>(base);
+        lmc = reinterpret_cast<ModeCoef*>(base + sizeof(BinData));
+    }
     GState<G> s;
     s.pass = 0;
     s.b = 0;
@@ -401,14 +414,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) 
             b = gshfl_i<G>(b, 0);
             if (b >= P.nbin) exhausted = true;
             else {
-                if constexpr (kStage) stage_tables(P, b, &lds_bd[wave], lds_mc[wave]);
+                if constexpr (kStage) stage_tables(P, b, lbd, lmc);
                 begin_binary<G>(P, s, b, gl);
             }
         }
         if (__all(exhausted)) break;
         if (!exhausted) {
             bool need_new;
-            if constexpr (kStage) iterate<G, true>(P, s, gl, need_new, &lds_bd[wave], lds_mc[wave]);
+            if constexpr (kStage) iterate<G, true>(P, s, gl, need_new, lbd, lmc);
             else iterate<G>(P, s, gl, need_new);
         }
     }
@@ -486,7 +499,7 @@ __global__ __launch_bounds__(kBlock) void k_plain_step(IntegParams P, PlainState
 }
 
 // (2,2) mode on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221)
-__global__ __launch_bounds__(kBlock) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
                                                         const ModeCoef* modes, int n, int max_step, double* knots) {
     const int b = blockIdx.y;
     if (b >= n) return;
@@ -517,7 +530,8 @@ int launch_persistent(const IntegParams& P, int block, int grid_groups, hipStrea
     const int groups_per_block = block / G;
     int nblk = (grid_groups + groups_per_block - 1) / groups_per_block;
     if (nblk < 1) nblk = 1;
-    hipLaunchKernelGGL(k_integrate_persistent<G>, dim3(nblk), dim3(block), 0, st, P);
+    const size_t lds = (G == 64) ? (size_t)(block / 64) * kStageBytesPerWave : 0;
+    hipLaunchKernelGGL(k_integrate_persistent<G>, dim3(nblk), dim3(block), lds, st, P);
     return check_launch("k_integrate_persistent");
 }
 
@@ -775,10 +789,15 @@ size_t eobhip_integrate_scratch_bytes(int n) {
     return 256 + nn * (6 * sizeof(int) + 12 * sizeof(double) + sizeof(unsigned long long) + 2 * sizeof(int)) + 16 * 256;
 }
 
-int eobhip_integrate(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol, double atol,
-                     double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step, const double* x0, const int* step0,
-                     int use_stop, int mode, int group, int block, int grid_groups, double* dense, int* num_steps, int* stop_info,
-                     int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out, void* stream) {
+}  // extern "C"
+
+namespace {
+
+int integrate_impl(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol, double atol,
+                   double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step, const double* x0,
+                   const int* step0, int use_stop, int mode, int group, int block, int grid_groups, double* dense, int* num_steps,
+                   int* stop_info, int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out, int* done,
+                   void* stream) {
     EOBHIP_REQUIRE((x0 == nullptr) == (step0 == nullptr), "x0 and step0 go together");
     EOBHIP_REQUIRE(n >= 0, "n < 0");
     EOBHIP_REQUIRE(max_step >= 2, "max_step must be >= 2");
@@ -810,6 +829,8 @@ int eobhip_integrate(const double* condBound, const void* bindata, const void* m
     P.x0 = x0;
     P.step0 = step0;
     P.use_stop = use_stop ? 1 : 0;
+    P.done = done;
+    EOBHIP_REQUIRE(done == nullptr || mode == 0, "completion flags need the persistent mode");
 
     // carve scratch
     char* base = static_cast<char*>(scratch);
@@ -865,6 +886,38 @@ int eobhip_integrate(const double* condBound, const void* bindata, const void* m
     }
     hipHostFree(host_cnt);
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int eobhip_integrate(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol, double atol,
+                     double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step, const double* x0, const int* step0,
+                     int use_stop, int mode, int group, int block, int grid_groups, double* dense, int* num_steps, int* stop_info,
+                     int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out, void* stream) {
+    return integrate_impl(condBound, bindata, modes, n, max_step, rtol, atol, hInit, step_back, fine_step, tmax, two_pass, fix_step, x0,
+                          step0, use_stop, mode, group, block, grid_groups, dense, num_steps, stop_info, pass1_steps, attempts, scratch,
+                          iterations_out, nullptr, stream);
+}
+
+int eobhip_integrate_signal(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol,
+                            double atol, double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step,
+                            const double* x0, const int* step0, int use_stop, int group, int block, int grid_groups, double* dense,
+                            int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts, void* scratch, int* done,
+                            void* stream) {
+    EOBHIP_REQUIRE(done != nullptr, "done is required");
+    // the device's address of the flags; fails (instead of a faulting store) for
+    // memory that is not pinned host memory
+    void* done_dev = nullptr;
+    const hipError_t e = hipHostGetDevicePointer(&done_dev, done, 0);
+    if (e != hipSuccess || done_dev == nullptr) {
+        set_error("done must be pinned host memory (hipHostGetDevicePointer: %s)", hipGetErrorString(e));
+        return EOBHIP_EINVAL;
+    }
+    return integrate_impl(condBound, bindata, modes, n, max_step, rtol, atol, hInit, step_back, fine_step, tmax, two_pass, fix_step, x0,
+                          step0, use_stop, 0, group, block, grid_groups, dense, num_steps, stop_info, pass1_steps, attempts, scratch,
+                          nullptr, static_cast<int*>(done_dev), stream);
 }
 
 int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n, int max_step,

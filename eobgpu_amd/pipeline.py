@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 
 import numpy as np
 import torch
@@ -123,12 +124,26 @@ class EOBBatch:
 
     # ------------------------------------------------------------- trajectory
     def integrate(self, rtol=1e-12, atol=1e-12, hInit=0.01, step_back=10.0, fine_step=0.05, two_pass=True, fix_step=False,
-                  mode="persistent", group=64, block=256, grid_groups=0, x0=None, step0=None, use_stop=True, tmax=1e13):
+                  mode="persistent", group=64, block=256, grid_groups=0, x0=None, step0=None, use_stop=True, tmax=1e13, done=None):
         """Run the batch.  Returns the driver iteration count (plain mode) or 0.
 
-        x0/step0 (device tensors [n]) continue an integration from row step0 of self.dense."""
+        x0/step0 (device tensors [n]) continue an integration from row step0 of self.dense.
+        done (pinned host int32 tensor [n], zeroed; persistent mode): done[b] becomes 1
+        when binary b's outputs are in memory (``eobhip_integrate_signal``)."""
         if self.cond is None:
             raise RuntimeError("initial state not set")
+        if done is not None:
+            if mode != "persistent":
+                raise ValueError("completion flags need the persistent mode")
+            if done.device.type != "cpu" or not done.is_pinned() or done.dtype != torch.int32 or done.numel() < self.n:
+                raise ValueError("done must be a pinned host int32 tensor of n elements")
+            call("eobhip_integrate_signal", self.cond.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n,
+                 self.max_step, float(rtol), float(atol), float(hInit), float(step_back), float(fine_step), float(tmax),
+                 int(bool(two_pass)), int(bool(fix_step)), None if x0 is None else x0.data_ptr(),
+                 None if step0 is None else step0.data_ptr(), int(bool(use_stop)), int(group), int(block), int(grid_groups),
+                 self.dense.data_ptr(), self.num_steps.data_ptr(), self.stop_info.data_ptr(), self.pass1_steps.data_ptr(),
+                 self.attempts.data_ptr(), self.scratch.data_ptr(), done.data_ptr(), current_stream())
+            return 0
         it = ctypes.c_longlong(0)
         call("eobhip_integrate", self.cond.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, self.max_step,
              float(rtol), float(atol), float(hInit), float(step_back), float(fine_step), float(tmax), int(bool(two_pass)),
@@ -271,6 +286,22 @@ class SupportLikelihood:
 
     def __call__(self, batch, total_mass_msun, dt, amp=None, out=None, terms=None):
         return self.complete(self.begin(batch, total_mass_msun, dt, amp), out=out, terms=terms)
+
+    def warm_transforms(self):
+        """Create the transform plans of every size (2^m_min .. N) and row chunk
+        (1, 2, 4, .. 64 rows) on the current stream -- plans are kept per stream
+        and size -- so that no later call on this stream spends host time
+        creating one (a call whose templates fall into new sizes or row counts
+        would otherwise)."""
+        dev, chunk = self.device, 64
+        src = torch.zeros(chunk << self.log2N, dtype=torch.float64, device=dev)
+        dst = torch.empty(chunk * ((self.N >> 1) + 1), dtype=torch.complex128, device=dev)
+        for k in range(self.m_min, self.log2N + 1):
+            rows = 1
+            while rows <= chunk:
+                call("eobhip_rfft_batched", src.data_ptr(), dst.data_ptr(), rows, 1 << k, current_stream())
+                rows *= 2
+        torch.cuda.current_stream(dev).synchronize()
 
     def begin(self, batch, total_mass_msun, dt, amp=None):
         """First half of a call: support sizes on the device and their copy to the
@@ -471,6 +502,176 @@ class ChunkedPass:
             with torch.cuda.stream(self.streams[k]):
                 self.out.index_copy_(0, self.idx[k], self.outs[k])
         for st in self.streams:
+            cur.wait_stream(st)
+        return self.out
+
+
+class _Rows:
+    """Downstream buffers for up to ``cap`` binaries gathered from a batch: the
+    rows, tables and per-binary inputs of the binaries being processed, the
+    knots and spline coefficients, and their log L.  ``view(k)`` is the first k
+    binaries in the shape ``EOBBatch``'s knot and spline calls and
+    ``SupportLikelihood`` expect."""
+
+    def __init__(self, cap, max_step, device):
+        L = lib()
+        self.cap, self.max_step, self.device = cap, max_step, device
+        self.bd_bytes, self.md_bytes = int(L.eobhip_bindata_bytes()), int(L.eobhip_modes_bytes())
+        self.dense = torch.empty((cap, max_step, 5), dtype=torch.float64, device=device)
+        self.num_steps = torch.empty(cap, dtype=torch.int32, device=device)
+        self.bindata = torch.empty((cap, self.bd_bytes), dtype=torch.uint8, device=device)
+        self.modes = torch.empty((cap, self.md_bytes), dtype=torch.uint8, device=device)
+        self.knots = torch.empty((cap, max_step, 3), dtype=torch.float64, device=device)
+        self.coef = torch.empty((cap, max_step, 12), dtype=torch.float64, device=device)
+        self.spline_scratch = torch.empty(int(L.eobhip_spline_scratch_doubles(cap, max_step)), dtype=torch.float64, device=device)
+        self.M = torch.empty(cap, dtype=torch.float64, device=device)
+        self.amp = torch.empty(cap, dtype=torch.float64, device=device)
+        self.pos = torch.empty(cap, dtype=torch.int64, device=device)
+        self.out = torch.empty(cap, dtype=torch.float64, device=device)
+        self.idx = torch.empty(cap, dtype=torch.int64, device=device)
+        self.idx_host = torch.empty(cap, dtype=torch.int64, pin_memory=True)
+
+    class _View:
+        hlm22_knots = EOBBatch.hlm22_knots
+        build_splines = EOBBatch.build_splines
+
+    def view(self, k):
+        v = self._View()
+        v.n, v.max_step, v.device = k, self.max_step, self.device
+        v.dense, v.num_steps = self.dense[:k], self.num_steps[:k]
+        v.bindata, v.modes = self.bindata[:k], self.modes[:k]
+        v.knots, v.coef, v.spline_scratch = self.knots[:k], self.coef[:k], self.spline_scratch
+        return v
+
+
+class StreamedPass:
+    """One pass over n binaries, like ``ChunkedPass``, with the downstream work
+    started per group of finished binaries instead of per chunk.
+
+    The whole batch, ordered longest expected first, integrates in one
+    persistent launch on its own stream (``eobhip_integrate_signal``), which
+    raises a pinned host flag per binary once the binary's rows are in memory.
+    The host polls the flags; whenever binaries have finished it gathers their
+    rows and tables into a downstream buffer (``_Rows``) and queues knots,
+    splines and the likelihood for them on one of two downstream streams (the
+    streams alternate, so one group's host-side likelihood bookkeeping
+    overlaps the other's kernels).  Downstream groups therefore size
+    themselves to the rate at which binaries finish, and what remains after
+    the last trajectory ends is the downstream work of the few binaries that
+    finished last.
+
+    Each binary's numbers are those of the one-batch pass (the integrator and
+    the knot, spline and likelihood kernels treat binaries independently; the
+    transforms run on other batch sizes, so log L may differ in the last
+    bits).  ``run`` blocks the calling thread until every binary's work is
+    queued.
+    """
+
+    def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, rtol=1e-10, device="cuda",
+                 order=None, poll_s=2e-5):
+        m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
+        n = m1.size
+        if n < 1:
+            raise ValueError("empty batch")
+        if order is None:
+            order = np.argsort(-expected_length(r0, m1, m2, chi1, chi2), kind="stable")   # longest first
+        order = np.asarray(order, dtype=np.int64)
+        if order.shape != (n,) or not np.array_equal(np.sort(order), np.arange(n)):
+            raise ValueError("order must be a permutation of range(n)")
+        self.n, self.lik, self.dt, self.rtol, self.device, self.poll_s = n, lik, float(dt), float(rtol), device, float(poll_s)
+        streams = concurrent_streams(3, device=device)
+        if len(streams) < 3:
+            raise RuntimeError(f"found {len(streams)} streams that run concurrently, the pass needs 3")
+        self.integ_stream, self.down_streams = streams[0], streams[1:]
+        self.order = order
+        M = np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))[order].copy()
+        A = np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (n,))[order].copy()
+        self.batch = EOBBatch(make_additional_args(m1[order], m2[order], chi1[order], chi2[order]), max_step=max_step, device=device)
+        self.r0 = torch.as_tensor(r0[order].copy(), device=device)
+        self.M = torch.as_tensor(M, device=device)
+        self.amp = torch.as_tensor(A, device=device)
+        self.pos = torch.as_tensor(order, device=device)
+        self.bd2 = self.batch.bindata.view(n, -1)
+        self.md2 = self.batch.modes.view(n, -1)
+        self.done = torch.zeros(n, dtype=torch.int32, pin_memory=True)
+        self.rows = [_Rows(n, self.batch.max_step, device) for _ in self.down_streams]
+        for st in self.down_streams:
+            with torch.cuda.stream(st):
+                lik.warm_transforms()
+        self.out = torch.empty(n, dtype=torch.float64, device=device)
+        self.groups = []   # caller indices of each downstream group of the last run
+        torch.cuda.synchronize(device)
+
+    def _launch(self, new, slot):
+        rows, st, k = self.rows[slot], self.down_streams[slot], new.size
+        rows.idx_host[:k].numpy()[:] = new
+        with torch.cuda.stream(st):
+            idx = rows.idx[:k]
+            idx.copy_(rows.idx_host[:k], non_blocking=True)
+            b = self.batch
+            torch.index_select(b.dense, 0, idx, out=rows.dense[:k])
+            torch.index_select(b.num_steps, 0, idx, out=rows.num_steps[:k])
+            torch.index_select(self.bd2, 0, idx, out=rows.bindata[:k])
+            torch.index_select(self.md2, 0, idx, out=rows.modes[:k])
+            torch.index_select(self.M, 0, idx, out=rows.M[:k])
+            torch.index_select(self.amp, 0, idx, out=rows.amp[:k])
+            torch.index_select(self.pos, 0, idx, out=rows.pos[:k])
+            v = rows.view(k)
+            v.hlm22_knots(out=v.knots)
+            v.build_splines()
+            h = self.lik.begin(v, rows.M[:k], self.dt, amp=rows.amp[:k])
+        return h, slot, k
+
+    def _complete(self, pend):
+        h, slot, k = pend
+        rows, st = self.rows[slot], self.down_streams[slot]
+        self.lik.complete(h, out=rows.out[:k])
+        with torch.cuda.stream(st):
+            self.out.index_copy_(0, rows.pos[:k], rows.out[:k])
+
+    def run(self, group=64, block=256, grid_groups=0, events=None):
+        """Launch the pass; returns log L [n] in the caller's order (valid on the
+        current stream once it returns).  ``events`` (optional) gets
+        'traj_end_longest' recorded when the integration ends."""
+        cur = torch.cuda.current_stream(self.device)
+        for st in (self.integ_stream, *self.down_streams):
+            st.wait_stream(cur)
+        done = self.done.numpy()
+        done[:] = 0   # the previous run's integrator raised every flag, so it writes none of these any more
+        b = self.batch
+        with torch.cuda.stream(self.integ_stream):
+            call("eobhip_bin_setup", b.add.data_ptr(), b.n, 1, b.bindata.data_ptr(), b.modes.data_ptr(), current_stream())
+            b.initial_conditions(self.r0)
+            b.integrate(rtol=self.rtol, atol=self.rtol, two_pass=True, group=group, block=block, grid_groups=grid_groups,
+                        done=self.done)
+            ended = torch.cuda.Event()
+            ended.record(self.integ_stream)
+            if events is not None:
+                events["traj_end_longest"].record(self.integ_stream)
+        taken = np.zeros(self.n, dtype=bool)
+        pending, slot = None, 0
+        self.groups = []
+        while True:
+            new = np.flatnonzero((done != 0) & ~taken)
+            if new.size:
+                taken[new] = True
+                self.groups.append(self.order[new])
+                nxt = self._launch(new, slot)
+                slot = (slot + 1) % len(self.down_streams)
+                if pending is not None:
+                    self._complete(pending)
+                pending = nxt
+                continue
+            if pending is not None and (taken.all() or pending[0]["ready"].query()):
+                self._complete(pending)
+                pending = None
+                continue
+            if taken.all():
+                break
+            if ended.query() and not (done != 0).all():
+                raise RuntimeError("the integrator ended without flagging every binary")
+            time.sleep(self.poll_s)
+        for st in (self.integ_stream, *self.down_streams):
             cur.wait_stream(st)
         return self.out
 

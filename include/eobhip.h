@@ -153,6 +153,22 @@ int eobhip_integrate(const double* condBound, const void* bindata, const void* m
                      double* dense, int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts,
                      void* scratch, long long* iterations_out, void* stream);
 
+/* eobhip_integrate in the persistent mode with a completion flag per binary:
+ * done[b] becomes 1 once binary b's rows, num_steps, stop_info, pass1_steps
+ * and attempts are in memory (a system-scope release, then the store), so a
+ * host thread polling done can start the binary's downstream work (knots,
+ * splines, likelihood) on another stream while the rest of the batch still
+ * integrates.  done: n ints in host memory the device can address (pinned,
+ * e.g. hipHostMalloc), zeroed by the caller before the launch.  No counterpart
+ * in the reference, whose driver returns after the whole batch
+ * (eob/new_Dormand_horiz.py:424-778); arguments as eobhip_integrate without
+ * mode (persistent) and iterations_out. */
+int eobhip_integrate_signal(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol,
+                            double atol, double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step,
+                            const double* x0, const int* step0, int use_stop, int group, int block, int grid_groups, double* dense,
+                            int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts, void* scratch, int* done,
+                            void* stream);
+
 /* (2,2) mode on the trajectory knots of every binary (get_hlms for (2,2)):
  * knots [n][max_step][3] = (phi, Re, Im) of h22 * exp(2 i phi); rows >= num_steps untouched */
 int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n,

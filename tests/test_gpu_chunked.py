@@ -1,4 +1,4 @@
-"""GPU: the stream-overlapped pass (ChunkedPass) gives every binary the numbers
+"""GPU: the stream-overlapped passes (ChunkedPass, StreamedPass) give every binary the numbers
 of the one-batch pass: the same trajectory, bit for bit (the integrator treats
 binaries independently), and the same log L up to the last bits (its
 transforms run at other batch sizes)."""
@@ -47,3 +47,54 @@ def test_chunked_pass_matches_one_batch(dev):
     # the longest expected binaries went to the first chunk
     key = P.expected_length(p["r0"], p["m1"], p["m2"], p["chi1"], p["chi2"])
     assert key[cp.groups[0]].min() >= key[cp.groups[-1]].max()
+
+
+def test_streamed_pass_matches_one_batch(dev):
+    """StreamedPass: one integrator launch that flags each finished binary, the
+    downstream work queued per group of finished binaries; the same numbers as
+    the one-batch pass."""
+    n = 24
+    p = configs.uneven_batch(n=n, seed=11)
+    p["r0"] = np.linspace(11.0, 17.0, n)
+    M = p["m1"] + p["m2"]
+    amp = np.linspace(0.5, 1.5, n)
+    rng = np.random.default_rng(5)
+    F = N // 2 + 1
+    data = (rng.normal(size=F) + 1j * rng.normal(size=F)) * 1e-2
+    lik = P.SupportLikelihood(data, N, 1.0 / (N * DT), m_min_log2=6)
+
+    b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=2048, device=dev)
+    b.initial_conditions(p["r0"])
+    b.integrate(rtol=1e-10, atol=1e-10, two_pass=True, block=64)
+    b.hlm22_knots()
+    b.build_splines()
+    ref = lik(b, M, DT, amp=amp).cpu().numpy()
+
+    sp = P.StreamedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], lik, M, DT, amp=amp, max_step=2048)
+    for _ in range(2):   # a second pass reuses the flags and the downstream buffers
+        got = sp.run().cpu().numpy()
+        torch.cuda.synchronize()
+        np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+        assert (sp.done.numpy() == 1).all()
+        taken = np.concatenate(sp.groups)
+        assert sorted(taken.tolist()) == list(range(n))   # every binary downstream exactly once
+    # the batch is the caller's binaries longest-expected first; trajectories bit for bit
+    key = P.expected_length(p["r0"], p["m1"], p["m2"], p["chi1"], p["chi2"])
+    assert np.all(np.diff(key[sp.order]) <= 0)
+    ns = sp.batch.num_steps.cpu().numpy()
+    np.testing.assert_array_equal(ns, b.num_steps.cpu().numpy()[sp.order])
+    L = int(ns.max())
+    np.testing.assert_array_equal(sp.batch.dense[:, :L].cpu().numpy(),
+                                  b.dense[torch.as_tensor(sp.order, device=dev)][:, :L].cpu().numpy())
+    np.testing.assert_array_equal(sp.batch.attempts.cpu().numpy(), b.attempts.cpu().numpy()[sp.order])
+
+
+def test_integrate_flags_need_pinned_memory(dev):
+    b = P.EOBBatch(P.make_additional_args([20.0], [10.0], [0.1], [0.0]), max_step=256, device=dev)
+    b.initial_conditions([12.0])
+    with pytest.raises(ValueError):
+        b.integrate(done=torch.zeros(1, dtype=torch.int32))            # pageable host memory
+    with pytest.raises(ValueError):
+        b.integrate(done=torch.zeros(1, dtype=torch.int32, device=dev))   # device memory
+    with pytest.raises(ValueError):
+        b.integrate(mode="plain", done=torch.zeros(1, dtype=torch.int32, pin_memory=True))
