@@ -129,6 +129,16 @@ __device__ __forceinline__ void start_pass(GState<G>& s, int pass, double h0) {
     s.k1_ok = 0;
 }
 
+// Wait here for this wavefront's outstanding vector-memory loads.  The state a
+// binary starts from comes from memory on the paths that (re)start it; without
+// this wait the compiler waits for those loads at the top of every iteration,
+// and on this architecture that wait also covers the rows stored since, so
+// every attempted step would stall until the previous step's stores are
+// acknowledged (slow while other kernels load the memory system).
+__device__ __forceinline__ void wait_loads() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);   // vmcnt(0); expcnt and lgkmcnt left at their maxima
+}
+
 template <int G>
 __device__ __forceinline__ void begin_binary(const IntegParams& P, GState<G>& s, int b, int gl) {
     s.b = b;
@@ -150,6 +160,7 @@ __device__ __forceinline__ void begin_binary(const IntegParams& P, GState<G>& s,
         s.x = 0.0;
         write_row(P, s, gl);
     }
+    wait_loads();
 }
 
 // relaxed device-coherent load (rows written earlier by this group)
@@ -198,6 +209,7 @@ __device__ __forceinline__ void restart_pass2(const IntegParams& P, GState<G>& s
     }
     s.step_num = best_i - 1;
     start_pass(s, 2, P.fine_step);
+    wait_loads();
 }
 
 template <int G>
