@@ -232,10 +232,24 @@ __device__ __forceinline__ void finish_binary(const IntegParams& P, GState<G>& s
     s.pass = 0;
 }
 
-// One attempted DOP853 step given k1 at the current state.
-template <int G>
+// Per stage, the twelve weights the stage loop needs after the right-hand side
+// (dStage.a[st][0..9], dStage.b[st], dStage.e[st]), for a copy in LDS: read
+// at the top of a stage, their latency hides behind the right-hand side (from
+// the constant table the compiler loads them just before their use and waits).
+constexpr int kStageRow = 12;
+constexpr int kStageRowsBytes = 12 * kStageRow * 8;
+__device__ __forceinline__ void stage_rows_copy(double* dst, int lane) {
+    for (int i = lane; i < 12 * kStageRow; i += 64) {
+        const int st = i / kStageRow, c = i % kStageRow;
+        dst[i] = (c < 10) ? dStage.a[st][c] : ((c == 10) ? dStage.b[st] : dStage.e[st]);
+    }
+}
+
+// One attempted DOP853 step given k1 at the current state.  rows: the stage
+// weights in LDS (stage_rows_copy) or null for the constant table.
+template <int G, bool kRows = false>
 __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s, const BinData& bd, const ModeCoef* __restrict__ mc,
-                                             const double k1f[4], int gl, bool fixed) {
+                                             const double k1f[4], int gl, bool fixed, const double* rows = nullptr) {
     constexpr int NC = GState<G>::NC;
     // Stages run in a loop that is not unrolled, so the right-hand side is
     // emitted once (a compact body that stays in the instruction cache).  Each
@@ -259,6 +273,19 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
     }
 #pragma unroll 1
     for (int st = 1; st < 12; ++st) {
+        double wa[10], wb, we;
+        if constexpr (kRows) {
+            const double* r = rows + st * kStageRow;
+#pragma unroll
+            for (int i = 0; i < 10; ++i) wa[i] = r[i];
+            wb = r[10];
+            we = r[11];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 10; ++i) wa[i] = dStage.a[st][i];
+            wb = dStage.b[st];
+            we = dStage.e[st];
+        }
         double arg[NC];
 #pragma unroll
         for (int q = 0; q < NC; ++q) arg[q] = s.y[q] + h * acc[0][q];
@@ -279,13 +306,13 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
         }
         double kk[4];
         group_rhs<G>(bd, mc, af[0], af[2], af[3], gl, kk);
-        const double bj = dStage.b[st], ej = dStage.e[st];
+        const double bj = wb, ej = we;
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
             const int c = comp_of<G>(gl, q);
             const double kj = kk[c < 4 ? c : 0];
 #pragma unroll
-            for (int i = 0; i < 10; ++i) acc[i][q] = acc[i + 1][q] + dStage.a[st][i] * kj;
+            for (int i = 0; i < 10; ++i) acc[i][q] = acc[i + 1][q] + wa[i] * kj;
             acc[10][q] = 0.0;
             bsum[q] = bsum[q] + bj * kj;
             esum[q] = esum[q] + ej * kj;
@@ -371,7 +398,8 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
 // One driver iteration for a group.  Returns false when the group needs a new
 // binary (persistent mode refills; plain mode marks it finished).
 // The tables come from HBM (P.bd, P.mc) or, when the caller staged them in LDS
-// for its binary (lds_bd / lds_mc non-null), from there.
+// for its binary (lds_bd / lds_mc non-null), from there; the LDS copy of the
+// stage weights (stage_rows_copy) then follows the modes' table.
 template <int G, bool kLds = false>
 __device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int gl, bool& need_new,
                                         const BinData* lds_bd = nullptr, const ModeCoef* lds_mc = nullptr) {
@@ -433,7 +461,8 @@ __device__ __forceinline__ void iterate(const IntegParams& P, GState<G>& s, int 
     }
     // pass 2 is always fixed-step; fix_step makes the first (or only) pass fixed-step too
     const bool fixed = (s.pass == 2) || P.fix_step;
-    attempt_step<G>(P, s, bd, mc, k1, gl, fixed);
+    if constexpr (kLds) attempt_step<G, true>(P, s, bd, mc, k1, gl, fixed, reinterpret_cast<const double*>(mc + kNumModes));
+    else attempt_step<G>(P, s, bd, mc, k1, gl, fixed);
 }
 
 }  // namespace eob

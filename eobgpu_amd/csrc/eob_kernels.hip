@@ -382,11 +382,12 @@ __device__ __forceinline__ void stage_tables(const IntegParams& P, int b, BinDat
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// LDS per wavefront of a 64-lane-group launch: its binary's tables.  The
+// LDS per wavefront of a 64-lane-group launch: its binary's tables and the
+// stage weights (stage_rows_copy).  The
 // launch sizes the dynamic LDS by the wavefronts of a block, so a one-wavefront
-// block holds one copy (about 11.5 KB) and leaves the rest of the CU's LDS to
+// block holds one copy (about 12.6 KB) and leaves the rest of the CU's LDS to
 // other kernels.
-constexpr size_t kStageBytesPerWave = sizeof(BinData) + kNumModes * sizeof(ModeCoef);
+constexpr size_t kStageBytesPerWave = sizeof(BinData) + kNumModes * sizeof(ModeCoef) + kStageRowsBytes;
 static_assert(sizeof(BinData) % 16 == 0 && kStageBytesPerWave % 16 == 0, "16-byte aligned tables");
 
 template <int G>
@@ -402,6 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) 
         unsigned char* base = stage_lds + (size_t)wave * kStageBytesPerWave;
         lbd = reinterpret_cast<BinData*>(base);
         lmc = reinterpret_cast<ModeCoef*>(base + sizeof(BinData));
+        stage_rows_copy(reinterpret_cast<double*>(lmc + kNumModes), threadIdx.x & 63);   // read after stage_tables' wave barrier
     }
     GState<G> s;
     s.pass = 0;
