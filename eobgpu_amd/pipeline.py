@@ -11,6 +11,7 @@ Every stage is a HIP kernel from libeobhip.so:
 """
 from __future__ import annotations
 
+import collections
 import ctypes
 import math
 import time
@@ -553,12 +554,13 @@ class StreamedPass:
     raises a pinned host flag per binary once the binary's rows are in memory.
     The host polls the flags; whenever binaries have finished it gathers their
     rows and tables into a downstream buffer (``_Rows``) and queues knots,
-    splines and the likelihood for them on one of two downstream streams (the
-    streams alternate, so one group's host-side likelihood bookkeeping
-    overlaps the other's kernels).  Downstream groups therefore size
-    themselves to the rate at which binaries finish, and what remains after
-    the last trajectory ends is the downstream work of the few binaries that
-    finished last.
+    splines and the first half of the likelihood for them on a free one of
+    the downstream streams (three by default); a group's second half (which needs its support
+    sizes on the host) is queued once they have arrived, oldest group first,
+    and the host never blocks on them while binaries may still finish.
+    Downstream groups therefore size themselves to the rate at which binaries
+    finish, and what remains after the last trajectory ends is the downstream
+    work of the few binaries that finished last.
 
     Each binary's numbers are those of the one-batch pass (the integrator and
     the knot, spline and likelihood kernels treat binaries independently; the
@@ -568,20 +570,26 @@ class StreamedPass:
     """
 
     def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, rtol=1e-10, device="cuda",
-                 order=None, poll_s=2e-5):
+                 order=None, poll_s=2e-5, down_streams=3):
         m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
         n = m1.size
         if n < 1:
             raise ValueError("empty batch")
+        if down_streams < 1:
+            raise ValueError("down_streams must be >= 1")
         if order is None:
             order = np.argsort(-expected_length(r0, m1, m2, chi1, chi2), kind="stable")   # longest first
         order = np.asarray(order, dtype=np.int64)
         if order.shape != (n,) or not np.array_equal(np.sort(order), np.arange(n)):
             raise ValueError("order must be a permutation of range(n)")
         self.n, self.lik, self.dt, self.rtol, self.device, self.poll_s = n, lik, float(dt), float(rtol), device, float(poll_s)
-        streams = concurrent_streams(3, device=device)
-        if len(streams) < 3:
-            raise RuntimeError(f"found {len(streams)} streams that run concurrently, the pass needs 3")
+        # one stream for the integrator and down_streams for the downstream groups
+        # (three by default: with the integrator's that makes the four hardware
+        # queues a process gets; the caller's stream, idle during the pass, shares
+        # one); fewer if fewer run concurrently, at least one
+        streams = concurrent_streams(1 + down_streams, device=device)
+        if len(streams) < 2:
+            raise RuntimeError(f"found {len(streams)} streams that run concurrently, the pass needs at least 2")
         self.integ_stream, self.down_streams = streams[0], streams[1:]
         self.order = order
         M = np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))[order].copy()
@@ -649,25 +657,31 @@ class StreamedPass:
             if events is not None:
                 events["traj_end_longest"].record(self.integ_stream)
         taken = np.zeros(self.n, dtype=bool)
-        pending, slot = None, 0
+        left = self.n
+        nslot = len(self.down_streams)
+        pending = collections.deque()   # launched groups whose second half is not queued yet, oldest first
         self.groups = []
         while True:
-            new = np.flatnonzero((done != 0) & ~taken)
-            if new.size:
-                taken[new] = True
-                self.groups.append(self.order[new])
-                nxt = self._launch(new, slot)
-                slot = (slot + 1) % len(self.down_streams)
-                if pending is not None:
-                    self._complete(pending)
-                pending = nxt
+            # the oldest group's second half once its sizes are on the host (never
+            # waiting for them while binaries may be finishing)
+            if pending and pending[0][0]["ready"].query():
+                self._complete(pending.popleft())
                 continue
-            if pending is not None and (taken.all() or pending[0]["ready"].query()):
-                self._complete(pending)
-                pending = None
+            if len(pending) < nslot:
+                new = np.flatnonzero((done != 0) & ~taken)
+                if new.size:
+                    taken[new] = True
+                    left -= new.size
+                    self.groups.append(self.order[new])
+                    busy = {pd[1] for pd in pending}
+                    slot = next(i for i in range(nslot) if i not in busy)
+                    pending.append(self._launch(new, slot))
+                    continue
+            if left == 0:
+                if not pending:
+                    break
+                self._complete(pending.popleft())
                 continue
-            if taken.all():
-                break
             if ended.query() and not (done != 0).all():
                 raise RuntimeError("the integrator ended without flagging every binary")
             time.sleep(self.poll_s)
