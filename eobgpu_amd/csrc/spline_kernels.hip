@@ -132,10 +132,14 @@ __host__ __device__ inline int spline_mmax(int max_step) {
     const int big = (max_step + kSplineThreads - 1) / kSplineThreads;
     return small > big ? small : big;
 }
-// per-interpolant scratch in doubles: slopes + per-chunk (c', d') rows
+// per-interpolant scratch in doubles: slopes, the system's rows (4 per knot)
+// and per-chunk (c', d') rows
 __host__ __device__ inline long long spline_scratch_per_interp(int max_step) {
-    return (long long)max_step + (long long)kSplineThreads * 2 * (spline_mmax(max_step) + 2 * kHalo);
+    return 5LL * max_step + (long long)kSplineThreads * 2 * (spline_mmax(max_step) + 2 * kHalo);
 }
+// rows whose loads a thread issues together in the serial sweeps: the sweeps
+// would otherwise wait one memory round trip per row
+constexpr int kSweepBatch = 8;
 
 __global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const double* __restrict__ dense, const double* __restrict__ knots,
                                                                      const int* __restrict__ num_steps, int n, int max_step,
@@ -158,31 +162,66 @@ __global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const doubl
     const int m = (len + P - 1) / P;
     const int mmax = spline_mmax(max_step);
     const int p = threadIdx.x;
+    // every row of the system (sub-diagonal, diagonal, super-diagonal, right
+    // side), by all threads at once
+    double* R = S + max_step;
+    for (int i = p; i < len; i += blockDim.x) {
+        int i0, i1, i2;
+        if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
+        else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
+        else { i0 = i - 1; i1 = i; i2 = i + 1; }
+        const Row r = nak_row(i, len, T[5 * i0], T[5 * i1], T[5 * i2], Y[3 * i0], Y[3 * i1], Y[3 * i2]);
+        R[4 * i] = r.ld;
+        R[4 * i + 1] = r.dg;
+        R[4 * i + 2] = r.ud;
+        R[4 * i + 3] = r.b;
+    }
+    __syncthreads();
     if (p < P) {
         const int s0 = p * m, e0 = (s0 + m < len) ? s0 + m : len;
         if (s0 < e0) {
             const int lo = (s0 - kHalo > 0) ? s0 - kHalo : 0;
             const int hi = (e0 + kHalo < len) ? e0 + kHalo : len;
-            double* cw = S + max_step + (size_t)p * 2 * (mmax + 2 * kHalo);   // c' and d' of rows lo..hi-1
+            double* cw = S + 5 * (size_t)max_step + (size_t)p * 2 * (mmax + 2 * kHalo);   // c' and d' of rows lo..hi-1
             double pc = 0.0, pd = 0.0;
-            for (int i = lo; i < hi; ++i) {
-                int i0, i1, i2;
-                if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
-                else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
-                else { i0 = i - 1; i1 = i; i2 = i + 1; }
-                const Row r = nak_row(i, len, T[5 * i0], T[5 * i1], T[5 * i2], Y[3 * i0], Y[3 * i1], Y[3 * i2]);
-                const double a = (i == lo) ? 0.0 : r.ld;          // cut below the halo
-                const double c = (i == hi - 1) ? 0.0 : r.ud;      // cut above the halo
-                const double den = r.dg - a * pc;
-                pc = c / den;
-                pd = (r.b - a * pd) / den;
-                cw[2 * (i - lo)] = pc;
-                cw[2 * (i - lo) + 1] = pd;
+            for (int i0 = lo; i0 < hi; i0 += kSweepBatch) {
+                double rl[kSweepBatch], rd[kSweepBatch], ru[kSweepBatch], rb[kSweepBatch];
+#pragma unroll
+                for (int j = 0; j < kSweepBatch; ++j) {
+                    const int i = (i0 + j < hi) ? i0 + j : hi - 1;   // in bounds; rows past hi are not used
+                    rl[j] = R[4 * i]; rd[j] = R[4 * i + 1]; ru[j] = R[4 * i + 2]; rb[j] = R[4 * i + 3];
+                }
+#pragma unroll
+                for (int j = 0; j < kSweepBatch; ++j) {
+                    const int i = i0 + j;
+                    if (i < hi) {
+                        const double a = (i == lo) ? 0.0 : rl[j];          // cut below the halo
+                        const double c = (i == hi - 1) ? 0.0 : ru[j];      // cut above the halo
+                        const double den = rd[j] - a * pc;
+                        pc = c / den;
+                        pd = (rb[j] - a * pd) / den;
+                        cw[2 * (i - lo)] = pc;
+                        cw[2 * (i - lo) + 1] = pd;
+                    }
+                }
             }
             double x = 0.0;
-            for (int i = hi - 1; i >= lo; --i) {
-                x = cw[2 * (i - lo) + 1] - cw[2 * (i - lo)] * x;
-                if (i >= s0 && i < e0) S[i] = x;
+            for (int i1 = hi - 1; i1 >= lo; i1 -= kSweepBatch) {
+                double wc[kSweepBatch], wd[kSweepBatch];
+#pragma unroll
+                for (int j = 0; j < kSweepBatch; ++j) {
+                    const int i = (i1 - j >= lo) ? i1 - j : lo;
+                    wc[j] = cw[2 * (i - lo)];
+                    wd[j] = cw[2 * (i - lo) + 1];
+                }
+#pragma unroll
+                for (int j = 0; j < kSweepBatch; ++j) {
+                    const int i = i1 - j;
+                    if (i >= lo) {
+                        x = wd[j] - wc[j] * x;
+                        if (i >= s0 && i < e0) S[i] = x;
+                    }
+                }
             }
         }
     }
