@@ -270,7 +270,8 @@ class SupportLikelihood:
         w = self._work.get(key)
         need = {"n": n, "h": max(ph, 1), "x": max(px, 1), "cross": n * gx, "hh": n * gx2}
         if w is None or any(w["cap"][k] < v for k, v in need.items()):
-            cap = {k: max(v, 0 if w is None else w["cap"][k]) for k, v in need.items()}
+            # grow by half again at least, so that calls of varying sizes reallocate rarely
+            cap = {k: max(v, 0 if w is None else w["cap"][k] + w["cap"][k] // 2) for k, v in need.items()}
             dev = self.device
             w = {"cap": cap,
                  "nsamp": torch.empty(cap["n"], dtype=torch.int32, device=dev),
@@ -566,11 +567,12 @@ class StreamedPass:
     the knot, spline and likelihood kernels treat binaries independently; the
     transforms run on other batch sizes, so log L may differ in the last
     bits).  ``run`` blocks the calling thread until every binary's work is
-    queued.
+    queued.  The constructor runs ``warm_passes`` passes (one by default):
+    the first pass of a new object allocates work buffers and loads kernels.
     """
 
     def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, rtol=1e-10, device="cuda",
-                 order=None, poll_s=2e-5, down_streams=3):
+                 order=None, poll_s=2e-5, down_streams=3, warm_passes=1):
         m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
         n = m1.size
         if n < 1:
@@ -609,6 +611,11 @@ class StreamedPass:
         self.out = torch.empty(n, dtype=torch.float64, device=device)
         self.groups = []   # caller indices of each downstream group of the last run
         torch.cuda.synchronize(device)
+        # the first pass allocates the likelihood's work buffers, loads the
+        # downstream kernels and fills the allocator's caches; run it here
+        for _ in range(int(warm_passes)):
+            self.run()
+            torch.cuda.synchronize(device)
 
     def _launch(self, new, slot):
         rows, st, k = self.rows[slot], self.down_streams[slot], new.size
