@@ -273,18 +273,15 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
     }
 #pragma unroll 1
     for (int st = 1; st < 12; ++st) {
-        double wa[10], wb, we;
+        // the LDS weights are read here, the constant table's where they are used
+        // (hoisting those as well costs the register-heavy narrow groups spills)
+        double wa[10] = {}, wb = 0.0, we = 0.0;
         if constexpr (kRows) {
             const double* r = rows + st * kStageRow;
 #pragma unroll
             for (int i = 0; i < 10; ++i) wa[i] = r[i];
             wb = r[10];
             we = r[11];
-        } else {
-#pragma unroll
-            for (int i = 0; i < 10; ++i) wa[i] = dStage.a[st][i];
-            wb = dStage.b[st];
-            we = dStage.e[st];
         }
         double arg[NC];
 #pragma unroll
@@ -306,13 +303,13 @@ __device__ __forceinline__ void attempt_step(const IntegParams& P, GState<G>& s,
         }
         double kk[4];
         group_rhs<G>(bd, mc, af[0], af[2], af[3], gl, kk);
-        const double bj = wb, ej = we;
+        const double bj = kRows ? wb : dStage.b[st], ej = kRows ? we : dStage.e[st];
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
             const int c = comp_of<G>(gl, q);
             const double kj = kk[c < 4 ? c : 0];
 #pragma unroll
-            for (int i = 0; i < 10; ++i) acc[i][q] = acc[i + 1][q] + wa[i] * kj;
+            for (int i = 0; i < 10; ++i) acc[i][q] = acc[i + 1][q] + (kRows ? wa[i] : dStage.a[st][i]) * kj;
             acc[10][q] = 0.0;
             bsum[q] = bsum[q] + bj * kj;
             esum[q] = esum[q] + ej * kj;
