@@ -27,6 +27,8 @@ _SIGS = {
     "eobhip_version": [],
     "eobhip_dop853_tableau": [c_vp],
     "eobhip_spin": [ctypes.c_longlong, c_vp],
+    "eobhip_gather_binaries": [c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, ctypes.c_size_t, c_vp, ctypes.c_size_t, c_vp, c_vp, c_vp,
+                               c_double, c_double, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
     "eobhip_evaluate_Ham_align_AD": [c_vp] * 9 + [c_double] * 4 + [c_int, c_vp],
     "eobhip_grad_Ham_align_AD": [c_vp, c_vp, c_vp, c_int, c_vp],
     "eobhip_hessian_Ham_align_AD": [c_vp, c_vp, c_vp, c_int, c_vp],

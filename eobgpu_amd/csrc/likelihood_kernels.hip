@@ -256,7 +256,58 @@ int log2_exact(long long N) {
 
 }  // namespace
 
+namespace {
+
+// Downstream gather: binary idx[j] of a batch (rows below its num_steps,
+// tables, amplitude, caller position) into slot j of a downstream buffer, with
+// dT_M[j] = dt / (M[idx[j]] MTSUN), rounded as torch evaluates dt / (M * MTSUN)
+// on the device (the product, its reciprocal, times dt).  One block per binary.
+__global__ __launch_bounds__(kBlock) void k_gather_binaries(const long long* __restrict__ idx, int k, int n_src, int max_step,
+                                                            const double* __restrict__ dense, const int* __restrict__ num_steps,
+                                                            const double* __restrict__ bd, int bd_words, const double* __restrict__ md,
+                                                            int md_words, const double* __restrict__ M, const double* __restrict__ amp,
+                                                            const long long* __restrict__ pos, double dt, double mtsun,
+                                                            double* __restrict__ dense_o, int* __restrict__ ns_o, double* __restrict__ bd_o,
+                                                            double* __restrict__ md_o, double* __restrict__ dTM_o, double* __restrict__ amp_o,
+                                                            long long* __restrict__ pos_o) {
+    const int j = blockIdx.x;
+    if (j >= k) return;
+    const long long b = idx[j];
+    if (b < 0 || b >= n_src) return;
+    const int len = num_steps[b];
+    const int rows = len < 0 ? 0 : (len > max_step ? max_step : len);
+    const double* src = dense + (size_t)b * max_step * 5;
+    double* dst = dense_o + (size_t)j * max_step * 5;
+    for (int i = threadIdx.x; i < rows * 5; i += blockDim.x) dst[i] = src[i];
+    for (int i = threadIdx.x; i < bd_words; i += blockDim.x) bd_o[(size_t)j * bd_words + i] = bd[(size_t)b * bd_words + i];
+    for (int i = threadIdx.x; i < md_words; i += blockDim.x) md_o[(size_t)j * md_words + i] = md[(size_t)b * md_words + i];
+    if (threadIdx.x == 0) {
+        ns_o[j] = len;
+        const double mm = M[b] * mtsun;
+        dTM_o[j] = (1.0 / mm) * dt;
+        amp_o[j] = amp[b];
+        pos_o[j] = pos[b];
+    }
+}
+
+}  // namespace
+
 extern "C" {
+
+int eobhip_gather_binaries(const long long* idx, int k, int n_src, int max_step, const double* dense, const int* num_steps,
+                           const void* bindata, size_t bindata_bytes, const void* modes, size_t modes_bytes, const double* total_mass,
+                           const double* amp, const long long* pos, double dt, double mtsun, double* dense_out, int* num_steps_out,
+                           void* bindata_out, void* modes_out, double* dT_M_out, double* amp_out, long long* pos_out, void* stream) {
+    EOBHIP_REQUIRE(k >= 0 && n_src >= 0 && max_step >= 1, "bad sizes");
+    EOBHIP_REQUIRE(bindata_bytes % 8 == 0 && modes_bytes % 8 == 0, "table sizes must be multiples of 8 bytes");
+    EOBHIP_REQUIRE(bindata_bytes / 8 <= 0x7fffffff && modes_bytes / 8 <= 0x7fffffff, "tables too large");
+    if (k == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_gather_binaries, dim3(k), dim3(kBlock), 0, as_stream(stream), idx, k, n_src, max_step, dense, num_steps,
+                       static_cast<const double*>(bindata), (int)(bindata_bytes / 8), static_cast<const double*>(modes),
+                       (int)(modes_bytes / 8), total_mass, amp, pos, dt, mtsun, dense_out, num_steps_out, static_cast<double*>(bindata_out),
+                       static_cast<double*>(modes_out), dT_M_out, amp_out, pos_out);
+    return check_launch("gather_binaries");
+}
 
 size_t eobhip_lik_bank_doubles(int N, int m_min_log2) {
     const int L = log2_exact(N);

@@ -277,8 +277,8 @@ class SupportLikelihood:
                  "nsamp": torch.empty(cap["n"], dtype=torch.int32, device=dev),
                  "mlog2": torch.empty(cap["n"], dtype=torch.int32, device=dev),
                  "mlog2_host": torch.empty(cap["n"], dtype=torch.int32, pin_memory=True),
-                 "offs_host": torch.empty((2, cap["n"]), dtype=torch.int64, pin_memory=True),
-                 "offs": torch.empty((2, cap["n"]), dtype=torch.int64, device=dev),
+                 "offs_host": torch.empty(2 * cap["n"], dtype=torch.int64, pin_memory=True),
+                 "offs": torch.empty(2 * cap["n"], dtype=torch.int64, device=dev),
                  "hbuf": torch.empty(cap["h"], dtype=torch.float64, device=dev),
                  "X": torch.empty(cap["x"], dtype=torch.complex128, device=dev),
                  "cross": torch.empty(max(cap["cross"], 1), dtype=torch.float64, device=dev),
@@ -305,14 +305,17 @@ class SupportLikelihood:
                 rows *= 2
         torch.cuda.current_stream(dev).synchronize()
 
-    def begin(self, batch, total_mass_msun, dt, amp=None):
+    def begin(self, batch, total_mass_msun, dt, amp=None, dT_M=None):
         """First half of a call: support sizes on the device and their copy to the
         host, queued on the current stream.  ``complete`` waits for that copy only,
         so a caller can queue the first halves of several batches (on their own
-        streams) before it waits for any of them."""
+        streams) before it waits for any of them.  dT_M (optional device tensor
+        [n]): the sample spacing in units of each binary's mass, dt / (M MTSUN),
+        when the caller has it (total_mass_msun is then not used)."""
         n, dev = batch.n, self.device
-        M = _device_vector(total_mass_msun, n, dev)
-        dT_M = (dt / (M * MTSUN_SI)).contiguous()
+        if dT_M is None:
+            M = _device_vector(total_mass_msun, n, dev)
+            dT_M = (dt / (M * MTSUN_SI)).contiguous()
         amp_t = torch.ones(n, dtype=torch.float64, device=dev) if amp is None else _device_vector(amp, n, dev)
         w = self._buffers(n, 0, 0, 0, 0)
         nsamp, mlog2 = w["nsamp"][:n], w["mlog2"][:n]
@@ -352,20 +355,21 @@ class SupportLikelihood:
         with torch.cuda.stream(st):
             sp = current_stream()
             w = self._buffers(n, ph, px, gx.value, gx2.value)
-            oh = w["offs_host"][:, :n]
-            oh[0].numpy()[:] = rowoff
-            oh[1].numpy()[:] = xoff
-            offs = w["offs"][:, :n]
+            # row offsets then transform offsets, one contiguous copy
+            oh = w["offs_host"][:2 * n]
+            oh[:n].numpy()[:] = rowoff
+            oh[n:].numpy()[:] = xoff
+            offs = w["offs"][:2 * n]
             offs.copy_(oh, non_blocking=True)
             hbuf, X, cross, hh = w["hbuf"], w["X"], w["cross"], w["hh"]
             call("eobhip_lik_resample", batch.dense.data_ptr(), batch.coef.data_ptr(), batch.num_steps.data_ptr(),
-                 h["amp"].data_ptr(), h["dT_M"].data_ptr(), nsamp.data_ptr(), mlog2.data_ptr(), offs[0].data_ptr(), n,
+                 h["amp"].data_ptr(), h["dT_M"].data_ptr(), nsamp.data_ptr(), mlog2.data_ptr(), offs[:n].data_ptr(), n,
                  batch.max_step, kmax, self.g.data_ptr(), hbuf.data_ptr(), cross.data_ptr(), sp)
             for Mk, h0, x0, cnt in buckets:
                 call("eobhip_rfft_batched", hbuf.data_ptr() + 8 * h0, X.data_ptr() + 16 * x0, int(cnt), int(Mk), sp)
             if out is None:
                 out = torch.empty(n, dtype=torch.float64, device=dev)
-            call("eobhip_lik_finish", X.data_ptr(), offs[1].data_ptr(), mlog2.data_ptr(), n, kmax, self.m_min, self.bank.data_ptr(),
+            call("eobhip_lik_finish", X.data_ptr(), offs[n:].data_ptr(), mlog2.data_ptr(), n, kmax, self.m_min, self.bank.data_ptr(),
                  cross.data_ptr(), hh.data_ptr(), self.dd.data_ptr(), self.df, out.data_ptr(),
                  None if terms is None else terms.data_ptr(), sp)
         self.nsamp, self.mlog2 = nsamp, mlog2
@@ -526,7 +530,7 @@ class _Rows:
         self.knots = torch.empty((cap, max_step, 3), dtype=torch.float64, device=device)
         self.coef = torch.empty((cap, max_step, 12), dtype=torch.float64, device=device)
         self.spline_scratch = torch.empty(int(L.eobhip_spline_scratch_doubles(cap, max_step)), dtype=torch.float64, device=device)
-        self.M = torch.empty(cap, dtype=torch.float64, device=device)
+        self.dT_M = torch.empty(cap, dtype=torch.float64, device=device)
         self.amp = torch.empty(cap, dtype=torch.float64, device=device)
         self.pos = torch.empty(cap, dtype=torch.int64, device=device)
         self.out = torch.empty(cap, dtype=torch.float64, device=device)
@@ -601,8 +605,6 @@ class StreamedPass:
         self.M = torch.as_tensor(M, device=device)
         self.amp = torch.as_tensor(A, device=device)
         self.pos = torch.as_tensor(order, device=device)
-        self.bd2 = self.batch.bindata.view(n, -1)
-        self.md2 = self.batch.modes.view(n, -1)
         self.done = torch.zeros(n, dtype=torch.int32, pin_memory=True)
         self.rows = [_Rows(n, self.batch.max_step, device) for _ in self.down_streams]
         for st in self.down_streams:
@@ -624,17 +626,15 @@ class StreamedPass:
             idx = rows.idx[:k]
             idx.copy_(rows.idx_host[:k], non_blocking=True)
             b = self.batch
-            torch.index_select(b.dense, 0, idx, out=rows.dense[:k])
-            torch.index_select(b.num_steps, 0, idx, out=rows.num_steps[:k])
-            torch.index_select(self.bd2, 0, idx, out=rows.bindata[:k])
-            torch.index_select(self.md2, 0, idx, out=rows.modes[:k])
-            torch.index_select(self.M, 0, idx, out=rows.M[:k])
-            torch.index_select(self.amp, 0, idx, out=rows.amp[:k])
-            torch.index_select(self.pos, 0, idx, out=rows.pos[:k])
+            # one launch (each host launch costs as much as a small group's kernels)
+            call("eobhip_gather_binaries", idx.data_ptr(), k, self.n, b.max_step, b.dense.data_ptr(), b.num_steps.data_ptr(),
+                 b.bindata.data_ptr(), rows.bd_bytes, b.modes.data_ptr(), rows.md_bytes, self.M.data_ptr(), self.amp.data_ptr(),
+                 self.pos.data_ptr(), self.dt, MTSUN_SI, rows.dense.data_ptr(), rows.num_steps.data_ptr(), rows.bindata.data_ptr(),
+                 rows.modes.data_ptr(), rows.dT_M.data_ptr(), rows.amp.data_ptr(), rows.pos.data_ptr(), current_stream())
             v = rows.view(k)
             v.hlm22_knots(out=v.knots)
             v.build_splines()
-            h = self.lik.begin(v, rows.M[:k], self.dt, amp=rows.amp[:k])
+            h = self.lik.begin(v, None, self.dt, amp=rows.amp[:k], dT_M=rows.dT_M[:k])
         return h, slot, k
 
     def _complete(self, pend):

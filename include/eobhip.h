@@ -211,6 +211,17 @@ size_t eobhip_lik_bank_doubles(int N, int m_min_log2);
 size_t eobhip_lik_setup_scratch_bytes(int N);
 int eobhip_lik_setup(const double* dtilde, int N, double df, double f0, int m_min_log2, double* g, double* dd, double* bank,
                      void* scratch, void* stream);
+/* Downstream gather: for j < k, binary idx[j] (< n_src) of a batch into slot j of
+ * downstream buffers of the same layouts: its dense rows below num_steps, num_steps,
+ * its bindata and modes tables (bindata_bytes and modes_bytes per binary, multiples
+ * of 8), amp and pos, and dT_M_out[j] = (1 / (total_mass[idx[j]] * mtsun)) * dt.  No
+ * reference counterpart: eobgpu_amd.pipeline.StreamedPass collects the binaries
+ * that have finished integrating with one launch. */
+int eobhip_gather_binaries(const long long* idx, int k, int n_src, int max_step, const double* dense, const int* num_steps,
+                           const void* bindata, size_t bindata_bytes, const void* modes, size_t modes_bytes, const double* total_mass,
+                           const double* amp, const long long* pos, double dt, double mtsun, double* dense_out, int* num_steps_out,
+                           void* bindata_out, void* modes_out, double* dT_M_out, double* amp_out, long long* pos_out, void* stream);
+
 /* Per template: support length nsamp[n] (as eobhip_resample_hplus counts it) and log2 M in mlog2[n]. */
 int eobhip_lik_sizes(const double* dense, const int* num_steps, const double* dT_M, int n, int max_step, int N, int m_min_log2,
                      int* nsamp, int* mlog2, void* stream);
