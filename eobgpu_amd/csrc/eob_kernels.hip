@@ -319,46 +319,56 @@ __global__ __launch_bounds__(kBlock) void k_bin_setup(const double* add, int n, 
     }
 }
 
-// full G = 1 RHS on precomputed tables (initial conditions, knots)
-__device__ __forceinline__ void rhs1(const BinData& bd, const ModeCoef* mc, double r, double pr, double L, double k[4]) {
-    group_rhs<1>(bd, mc, r, pr, L, 0, k);
-}
-
-__device__ double ic_diss(const BinData& bd, const ModeCoef* mc, double r, double pr, double L) {
+// The dissipative initial condition (IC_diss, src/EOB.cu:2666): dr/dt from the
+// pphi loss along circular orbits, dr/dt = (dpphi/dt) / (dpphi/dr), minus the
+// right-hand side's dr/dt, with dpphi/dr = -H_rr / H_rpphi; evaluated with the
+// right-hand side of a 64-lane group (one flux mode per lane); every lane
+// returns the same value (lane 0's k: lane 63 evaluates H at pr = 0).
+__device__ double ic_diss64(const BinData& bd, const ModeCoef* mc, double r, double pr, double L, int gl) {
     double k[4];
-    rhs1(bd, mc, r, pr, L, k);
+    group_rhs<64>(bd, mc, r, pr, L, gl, k);
+    const double k0 = read_lane(k[0], 0), k3 = read_lane(k[3], 0);
     const double hrr = ham_second(bd.h, r, pr, L, 0, 0);
     const double hrL = ham_second(bd.h, r, pr, L, 0, 3);
     const double dLdr = -hrr / hrL;
-    const double rdot = k[3] / dLdr;
-    return rdot - k[0];
+    const double rdot = k3 / dLdr;
+    return rdot - k0;
 }
 
+// Circular initial conditions (IC_cons / IC_diss), one wavefront per binary:
+// Newton for pphi (every lane the same values), then the reference's false
+// position for pr on the dissipative condition.  One binary per lane took the
+// 35 flux modes of every evaluation in turn; this kernel starts every pass, so
+// its latency is on the pass's critical path.
 __global__ __launch_bounds__(64) void k_initial_conditions(const double* r0, const BinData* bdp, const ModeCoef* modes, int n,
                                                            double* pphi0, double* pr0, double* omega0, int max_iter, double err) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const BinData bd = bdp[i];
-        const ModeCoef* mc = modes + (size_t)i * kNumModes;
-        const double r = r0[i];
-        double L = sqrt(r);
-        for (int it = 0; it < 100; ++it) {
-            const HDual R(r, 1.0, 0.0, 0.0), PR(0.0), LL(L, 0.0, 1.0, 0.0);
-            const HDual H = ham_eval<HDual>(bd.h, R, PR, LL);
-            const double dL = H.a / H.ab;
-            L -= dL;
-            if (fabs(dL) < 1e-15 * fabs(L)) break;
-        }
-        omega0[i] = ham_tangent(bd.h, r, 0.0, L, 2).d;
-        double a = -3e-2, b = 0.0;
-        double fa = ic_diss(bd, mc, r, a, L), fb = ic_diss(bd, mc, r, b, L);
-        double c = 0.0;
-        for (int it = 0; it < max_iter; ++it) {
-            c = (a * fb - b * fa) / (fb - fa);
-            const double fc = ic_diss(bd, mc, r, c, L);
-            if ((fb < 0.0 && fc < 0.0) || (fb >= 0.0 && fc >= 0.0)) { b = c; fb = fc; }
-            else { a = c; fa = fc; }
-            if (fabs(fc) < err) break;
-        }
+    const int i = blockIdx.x;
+    const int gl = threadIdx.x;
+    if (i >= n) return;
+    const BinData& bd = bdp[i];
+    const ModeCoef* mc = modes + (size_t)i * kNumModes;
+    const double r = r0[i];
+    double L = sqrt(r);
+    for (int it = 0; it < 100; ++it) {
+        const HDual R(r, 1.0, 0.0, 0.0), PR(0.0), LL(L, 0.0, 1.0, 0.0);
+        const HDual H = ham_eval<HDual>(bd.h, R, PR, LL);
+        const double dL = H.a / H.ab;
+        L -= dL;
+        if (fabs(dL) < 1e-15 * fabs(L)) break;
+    }
+    const double om = ham_tangent(bd.h, r, 0.0, L, 2).d;
+    double a = -3e-2, b = 0.0;
+    double fa = ic_diss64(bd, mc, r, a, L, gl), fb = ic_diss64(bd, mc, r, b, L, gl);
+    double c = 0.0;
+    for (int it = 0; it < max_iter; ++it) {
+        c = (a * fb - b * fa) / (fb - fa);
+        const double fc = ic_diss64(bd, mc, r, c, L, gl);
+        if ((fb < 0.0 && fc < 0.0) || (fb >= 0.0 && fc >= 0.0)) { b = c; fb = fc; }
+        else { a = c; fa = fc; }
+        if (fabs(fc) < err) break;
+    }
+    if (gl == 0) {
+        omega0[i] = om;
         pphi0[i] = L;
         pr0[i] = c;
     }
@@ -779,7 +789,7 @@ int eobhip_initial_conditions(const double* r0, const void* bindata, const void*
                               double* omega0, int max_iter, double err, void* stream) {
     EOBHIP_REQUIRE(n >= 0 && max_iter >= 1, "bad arguments");
     if (n == 0) return EOBHIP_OK;
-    hipLaunchKernelGGL(k_initial_conditions, dim3(blocks_for(n, 64)), dim3(64), 0, as_stream(stream), r0,
+    hipLaunchKernelGGL(k_initial_conditions, dim3(n), dim3(64), 0, as_stream(stream), r0,
                        static_cast<const BinData*>(bindata), static_cast<const ModeCoef*>(modes), n, pphi0, pr0, omega0, max_iter, err);
     return check_launch("initial_conditions");
 }
