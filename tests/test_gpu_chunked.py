@@ -98,3 +98,37 @@ def test_integrate_flags_need_pinned_memory(dev):
         b.integrate(done=torch.zeros(1, dtype=torch.int32, device=dev))   # device memory
     with pytest.raises(ValueError):
         b.integrate(mode="plain", done=torch.zeros(1, dtype=torch.int32, pin_memory=True))
+
+
+def test_gather_binaries(dev):
+    """eobhip_gather_binaries against torch's index_select on a small batch (rows
+    below num_steps, tables, amplitudes, positions) and dT_M against the host's
+    expression, bit for bit."""
+    n, ms = 6, 128
+    p = configs.uneven_batch(n=n, seed=3)
+    b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=ms, device=dev)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    b.dense.copy_(torch.randn(b.dense.shape, generator=gen, dtype=torch.float64))
+    b.num_steps.copy_(torch.tensor([5, 128, 1, 77, 30, 2], dtype=torch.int32))
+    M = torch.as_tensor(p["m1"] + p["m2"], device=dev)
+    amp = torch.linspace(0.5, 2.0, n, dtype=torch.float64, device=dev)
+    pos = torch.arange(100, 100 + n, dtype=torch.int64, device=dev)
+    rows = P._Rows(n, ms, dev)
+    idx = torch.tensor([4, 1, 3], dtype=torch.int64, device=dev)
+    k, dt = idx.numel(), 1.0 / 4096
+    rows.dense.zero_()
+    P.call("eobhip_gather_binaries", idx.data_ptr(), k, n, ms, b.dense.data_ptr(), b.num_steps.data_ptr(), b.bindata.data_ptr(),
+           rows.bd_bytes, b.modes.data_ptr(), rows.md_bytes, M.data_ptr(), amp.data_ptr(), pos.data_ptr(), dt, P.MTSUN_SI,
+           rows.dense.data_ptr(), rows.num_steps.data_ptr(), rows.bindata.data_ptr(), rows.modes.data_ptr(), rows.dT_M.data_ptr(),
+           rows.amp.data_ptr(), rows.pos.data_ptr(), P.current_stream())
+    torch.cuda.synchronize()
+    ns = b.num_steps[idx].cpu().numpy()
+    np.testing.assert_array_equal(rows.num_steps[:k].cpu().numpy(), ns)
+    for j in range(k):
+        np.testing.assert_array_equal(rows.dense[j, :ns[j]].cpu().numpy(), b.dense[idx[j], :ns[j]].cpu().numpy())
+        assert not rows.dense[j, ns[j]:].any()   # rows at and past num_steps are not copied
+    np.testing.assert_array_equal(rows.bindata[:k].cpu().numpy(), b.bindata.view(n, -1)[idx].cpu().numpy())
+    np.testing.assert_array_equal(rows.modes[:k].cpu().numpy(), b.modes.view(n, -1)[idx].cpu().numpy())
+    np.testing.assert_array_equal(rows.amp[:k].cpu().numpy(), amp[idx].cpu().numpy())
+    np.testing.assert_array_equal(rows.pos[:k].cpu().numpy(), pos[idx].cpu().numpy())
+    np.testing.assert_array_equal(rows.dT_M[:k].cpu().numpy(), (dt / (M * P.MTSUN_SI))[idx].cpu().numpy())
