@@ -632,9 +632,12 @@ class StreamedPass:
                  self.pos.data_ptr(), self.dt, MTSUN_SI, rows.dense.data_ptr(), rows.num_steps.data_ptr(), rows.bindata.data_ptr(),
                  rows.modes.data_ptr(), rows.dT_M.data_ptr(), rows.amp.data_ptr(), rows.pos.data_ptr(), current_stream())
             v = rows.view(k)
+            # the support sizes need only the rows: queued before the knots and
+            # splines, they reach the host while those run, and the second half
+            # (queued after them on this stream) is ready to go when they end
+            h = self.lik.begin(v, None, self.dt, amp=rows.amp[:k], dT_M=rows.dT_M[:k])
             v.hlm22_knots(out=v.knots)
             v.build_splines()
-            h = self.lik.begin(v, None, self.dt, amp=rows.amp[:k], dT_M=rows.dT_M[:k])
         return h, slot, k
 
     def _complete(self, pend):
