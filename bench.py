@@ -37,8 +37,8 @@ sys.path.insert(0, ROOT)
 STEP_FLOPS = 50663
 # HBM traffic of one integrator launch on the headline batch (rocprofv3 FETCH_SIZE
 # and WRITE_SIZE, each scaled by the counter's reading for a 1 GiB device copy;
-# profiles/pipeline_uneven1024.md): 28.68 MB read + 27.77 MB written.
-INTEGRATOR_TRAFFIC_BYTES = (28.68 + 27.77) * 1e6
+# profiles/pipeline_uneven1024.md): 28.70 MB read + 27.81 MB written.
+INTEGRATOR_TRAFFIC_BYTES = (28.70 + 27.81) * 1e6
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
