@@ -62,8 +62,39 @@ struct WaveCoef {
 // prefixes: 70 doubles (Re, Im per mode) in flux-mode order.
 // Inputs follow SEOBNRv5::update_information (src/EOB.cu:3355-3374): eta, the
 // Kerr-like spin a (tplspin), chiS, chiA.  prefixes may be null (N2 left 0).
+// The part of mode i's table that does not depend on the binary's PN
+// coefficients: (l, m), 1/(l!)^2, the product polynomial, ln 2m, m^2,
+// |prefix Y|^2, and the coefficients the tables below set, zeroed.
+EOB_HD void flux_mode_init(ModeCoef& z, int i, double pre_re, double pre_im, bool have_prefix) {
+    for (int k = 0; k < 11; ++k) z.c[k] = 0.0;
+    z.cl6 = z.cl7 = z.cl8 = z.cl10 = 0.0;
+    z.f1 = z.f3 = 0.0;
+    int l, m;
+    mode_lm(i, l, m);
+    z.l = l; z.m = m;
+    const int eps = (l + m) & 1;
+    z.lpe = l + eps;
+    double lf = 1.0;
+    for (int q = 2; q <= l; ++q) lf *= q;
+    z.inv_lfac2 = 1.0 / (lf * lf);
+    // expand prod_{i=1..l} (h + i^2) one factor at a time (exact: the
+    // coefficients stay below 2^53)
+    for (int j = 0; j < 9; ++j) z.pp[j] = (j == 0) ? 1.0 : 0.0;
+    for (int q = 1; q <= l; ++q) {
+        for (int j = 8; j >= 1; --j) z.pp[j] = z.pp[j - 1] + (double)(q * q) * z.pp[j];
+        z.pp[0] *= (double)(q * q);
+    }
+    z.log2m = log(2.0 * m);
+    z.m2 = (double)(m * m);
+    const double pa2 = have_prefix ? pre_re * pre_re + pre_im * pre_im : 0.0;
+    const double y = ylm_abs_eq(l - eps, m);
+    z.N2 = pa2 * y * y;
+}
+
+// tables_only: the caller has run flux_mode_init on every mode (one mode per
+// lane); only the PN coefficient tables and the (2,2) waveform terms are set.
 EOB_HD void flux_setup(ModeCoef* mc, WaveCoef& wc, double m1, double m2, double eta, double a, double chiS, double chiA,
-                       const double* prefixes, double h22_calib, int use_hm) {
+                       const double* prefixes, double h22_calib, int use_hm, bool tables_only = false) {
     const double PI = kPi, PI2 = kPi * kPi;
     const double eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta2 * eta2;
     double dM2 = 1.0 - 4.0 * eta;
@@ -80,32 +111,9 @@ EOB_HD void flux_setup(ModeCoef* mc, WaveCoef& wc, double m1, double m2, double 
     const double p6 = -1.0 + 6.0 * eta - 10.0 * eta2 + 4.0 * eta3;
     const double q3 = dM2 + 3.0 * eta2;
 
-    for (int i = 0; i < kNumModes; ++i) {
-        ModeCoef& z = mc[i];
-        for (int k = 0; k < 11; ++k) z.c[k] = 0.0;
-        z.cl6 = z.cl7 = z.cl8 = z.cl10 = 0.0;
-        z.f1 = z.f3 = 0.0;
-        int l, m;
-        mode_lm(i, l, m);
-        z.l = l; z.m = m;
-        const int eps = (l + m) & 1;
-        z.lpe = l + eps;
-        double lf = 1.0;
-        for (int q = 2; q <= l; ++q) lf *= q;
-        z.inv_lfac2 = 1.0 / (lf * lf);
-        // expand prod_{i=1..l} (h + i^2) one factor at a time (exact: the
-        // coefficients stay below 2^53)
-        for (int j = 0; j < 9; ++j) z.pp[j] = (j == 0) ? 1.0 : 0.0;
-        for (int q = 1; q <= l; ++q) {
-            for (int j = 8; j >= 1; --j) z.pp[j] = z.pp[j - 1] + (double)(q * q) * z.pp[j];
-            z.pp[0] *= (double)(q * q);
-        }
-        z.log2m = log(2.0 * m);
-        z.m2 = (double)(m * m);
-        const double pa2 = prefixes ? prefixes[2 * i] * prefixes[2 * i] + prefixes[2 * i + 1] * prefixes[2 * i + 1] : 0.0;
-        const double y = ylm_abs_eq(l - eps, m);
-        z.N2 = pa2 * y * y;
-    }
+    if (!tables_only)
+        for (int i = 0; i < kNumModes; ++i)
+            flux_mode_init(mc[i], i, prefixes ? prefixes[2 * i] : 0.0, prefixes ? prefixes[2 * i + 1] : 0.0, prefixes != nullptr);
     // rho(v) = 1 + sum c[k-1] v^k; helpers name the reference's coefficient symbols
 #define RHO(idx, k) mc[idx].c[(k) - 1]
     // (2,2) -- mode 1

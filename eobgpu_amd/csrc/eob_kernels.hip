@@ -290,32 +290,42 @@ __global__ __launch_bounds__(kBlock) void k_controller(unsigned char* flag, cons
 }
 
 // ------------------------------------------------------------ batch pipeline
-__global__ __launch_bounds__(kBlock) void k_bin_setup(const double* add, int n, int use_hm, BinData* bd, ModeCoef* modes) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double m1 = add[i], m2 = add[n + i], c1 = add[2 * n + i], c2 = add[3 * n + i];
-        BinData d;
+// Tables of one binary per wavefront: lane q < 35 sets up flux mode q's
+// binary-independent part (flux_mode_init, with its flags), then lane 0 the
+// Hamiltonian coefficients, the modes' PN coefficient tables and the 13 pieces,
+// straight into the binary's BinData.  With one binary per lane the 35 modes
+// ran in turn; this kernel starts every pass, so its latency is on the pass's
+// critical path.
+__global__ __launch_bounds__(64) void k_bin_setup(const double* add, int n, int use_hm, BinData* bd, ModeCoef* modes) {
+    const int i = blockIdx.x, lane = threadIdx.x;
+    if (i >= n) return;
+    const double m1 = add[i], m2 = add[n + i], c1 = add[2 * n + i], c2 = add[3 * n + i];
+    const double nu = nu_of(m1, m2);
+    double ef = nu;
+    if (ef > 0.25 && ef < 0.25 + 1e-4) ef = 0.25;
+    ModeCoef* mc = modes + (size_t)i * kNumModes;
+    if (lane < kNumModes) {
+        ModeCoef& z = mc[lane];
+        flux_mode_init(z, lane, add[(10 + 2 * lane) * n + i], add[(11 + 2 * lane) * n + i], true);
+        // the flux's mode flags follow the flux eta
+        int fl = ((z.l + z.m) & 1) ? 0 : 1;
+        if (ef == 0.25 && (z.m & 1)) fl |= 2;
+        z.flags = fl;
+    }
+    __syncthreads();   // lane 0's tables below overwrite coefficients the modes' lanes zeroed
+    if (lane == 0) {
+        BinData& d = bd[i];
         ham_setup(d.h, m1, m2, c1, c2, add[5 * n + i], add[6 * n + i], add[7 * n + i], add[8 * n + i]);
-        d.nu = nu_of(m1, m2);
-        d.flux_scale = 1.0 / (8.0 * kPi * d.nu);
-        double ef = d.nu;
-        if (ef > 0.25 && ef < 0.25 + 1e-4) ef = 0.25;
+        d.nu = nu;
+        d.flux_scale = 1.0 / (8.0 * kPi * nu);
         d.eta_f = ef;
         // setup_class inputs (eob/seobnrv4phm.py:1924-1929)
         const double eta = m1 * m2 / ((m1 + m2) * (m1 + m2));
         const double chiS = 0.5 * (c1 + c2), chiA = 0.5 * (c1 - c2);
         const double a = (1 - 2 * eta) * chiS + (m1 - m2) / (m1 + m2) * chiA;
-        double pf[70];
-        for (int q = 0; q < 70; ++q) pf[q] = add[(10 + q) * n + i];
-        ModeCoef* mc = modes + (size_t)i * kNumModes;
-        flux_setup(mc, d.w, m1, m2, eta, a, chiS, chiA, pf, add[9 * n + i], use_hm);
-        // the flux's mode flags follow the flux eta
-        for (int q = 0; q < kNumModes; ++q) {
-            int fl = ((mc[q].l + mc[q].m) & 1) ? 0 : 1;
-            if (ef == 0.25 && (mc[q].m & 1)) fl |= 2;
-            mc[q].flags = fl;
-        }
-        bd[i] = d;
-        piece_setup(bd[i].pc, d.h);
+        const double pf22[4] = {add[10 * n + i], add[11 * n + i], add[12 * n + i], add[13 * n + i]};   // (2,1) and (2,2) prefixes
+        flux_setup(mc, d.w, m1, m2, eta, a, chiS, chiA, pf22, add[9 * n + i], use_hm, true);
+        piece_setup(d.pc, d.h);
     }
 }
 
@@ -510,7 +520,10 @@ __global__ __launch_bounds__(kBlock) void k_plain_step(IntegParams P, PlainState
     }
 }
 
-// (2,2) mode on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221)
+// (2,2) mode on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221).
+// Registers capped at three wavefronts per SIMD (168, a few spills): with the
+// 284 it would take, a wavefront could not share a SIMD with an integrating one
+// (292 of 512), so knots of finished binaries waited for whole free SIMDs.
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
                                                         const ModeCoef* modes, int n, int max_step, double* knots) {
     const int b = blockIdx.y;
@@ -780,7 +793,7 @@ size_t eobhip_modes_bytes(void) { return sizeof(ModeCoef) * kNumModes; }
 int eobhip_bin_setup(const double* add, int n, int use_hm, void* bindata, void* modes, void* stream) {
     EOBHIP_REQUIRE(n >= 0, "n < 0");
     if (n == 0) return EOBHIP_OK;
-    hipLaunchKernelGGL(k_bin_setup, dim3(blocks_for(n, 64)), dim3(64), 0, as_stream(stream), add, n, use_hm,
+    hipLaunchKernelGGL(k_bin_setup, dim3(n), dim3(64), 0, as_stream(stream), add, n, use_hm,
                        static_cast<BinData*>(bindata), static_cast<ModeCoef*>(modes));
     return check_launch("bin_setup");
 }
