@@ -550,6 +550,51 @@ class _Rows:
         return v
 
 
+def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, poll_s=2e-5, spin_below=32):
+    """The host's schedule in a streamed pass.
+
+    done: host int array the integrator fills (nonzero = finished).
+    launch(new, slot) queues the first half of the group of newly finished
+    binaries ``new`` on downstream slot ``slot`` and returns a handle;
+    is_ready(handle) says whether that group's support sizes are on the host;
+    complete(handle) queues its second half.  Second halves are queued oldest
+    group first; at most ``nslot`` groups are in flight and a slot is given
+    to a new group only once its previous group's second half is queued; the
+    host never waits for a group's sizes while binaries may still finish.
+    Polls sleep ``poll_s`` while more than ``spin_below`` binaries remain.
+    Raises if ``integrator_ended()`` while flags are missing.  Returns the
+    groups (index arrays into done) in launch order."""
+    n = done.size
+    taken = np.zeros(n, dtype=bool)
+    left = n
+    pending = collections.deque()   # (handle, slot) of groups whose second half is not queued yet, oldest first
+    groups = []
+    while True:
+        if pending and is_ready(pending[0][0]):
+            complete(pending.popleft()[0])
+            continue
+        if len(pending) < nslot:
+            new = np.flatnonzero((done != 0) & ~taken)
+            if new.size:
+                taken[new] = True
+                left -= new.size
+                groups.append(new)
+                busy = {sl for _, sl in pending}
+                slot = next(i for i in range(nslot) if i not in busy)
+                pending.append((launch(new, slot), slot))
+                continue
+        if left == 0:
+            if not pending:
+                break
+            complete(pending.popleft()[0])   # nothing else left to do: wait for its sizes
+            continue
+        if integrator_ended() and not (done != 0).all():
+            raise RuntimeError("the integrator ended without flagging every binary")
+        if left > spin_below:
+            time.sleep(poll_s)   # the last few binaries are polled without sleeping (a sleep lasts 50 us or more)
+    return groups
+
+
 class StreamedPass:
     """One pass over n binaries, like ``ChunkedPass``, with the downstream work
     started per group of finished binaries instead of per chunk.
@@ -666,36 +711,9 @@ class StreamedPass:
             ended.record(self.integ_stream)
             if events is not None:
                 events["traj_end_longest"].record(self.integ_stream)
-        taken = np.zeros(self.n, dtype=bool)
-        left = self.n
-        nslot = len(self.down_streams)
-        pending = collections.deque()   # launched groups whose second half is not queued yet, oldest first
-        self.groups = []
-        while True:
-            # the oldest group's second half once its sizes are on the host (never
-            # waiting for them while binaries may be finishing)
-            if pending and pending[0][0]["ready"].query():
-                self._complete(pending.popleft())
-                continue
-            if len(pending) < nslot:
-                new = np.flatnonzero((done != 0) & ~taken)
-                if new.size:
-                    taken[new] = True
-                    left -= new.size
-                    self.groups.append(self.order[new])
-                    busy = {pd[1] for pd in pending}
-                    slot = next(i for i in range(nslot) if i not in busy)
-                    pending.append(self._launch(new, slot))
-                    continue
-            if left == 0:
-                if not pending:
-                    break
-                self._complete(pending.popleft())
-                continue
-            if ended.query() and not (done != 0).all():
-                raise RuntimeError("the integrator ended without flagging every binary")
-            if left > 32:
-                time.sleep(self.poll_s)   # the last few binaries are polled without sleeping (a sleep lasts 50 us or more)
+        groups = drain_finished(done, len(self.down_streams), self._launch, self._complete, lambda h: h[0]["ready"].query(),
+                                ended.query, self.poll_s)
+        self.groups = [self.order[g] for g in groups]
         for st in (self.integ_stream, *self.down_streams):
             cur.wait_stream(st)
         return self.out
