@@ -48,6 +48,7 @@ _SIGS = {
     "eobhip_modes_bytes": [],
     "eobhip_bin_setup": [c_vp, c_int, c_int, c_vp, c_vp, c_vp],
     "eobhip_initial_conditions": [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_double, c_vp],
+    "eobhip_initial_state": [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_double, c_vp],
     "eobhip_integrate_scratch_bytes": [c_int],
     "eobhip_integrate": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
                          c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,

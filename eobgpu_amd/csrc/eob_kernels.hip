@@ -351,7 +351,8 @@ __device__ double ic_diss64(const BinData& bd, const ModeCoef* mc, double r, dou
 // 35 flux modes of every evaluation in turn; this kernel starts every pass, so
 // its latency is on the pass's critical path.
 __global__ __launch_bounds__(64) void k_initial_conditions(const double* r0, const BinData* bdp, const ModeCoef* modes, int n,
-                                                           double* pphi0, double* pr0, double* omega0, int max_iter, double err) {
+                                                           double* pphi0, double* pr0, double* omega0, int max_iter, double err,
+                                                           double* cond) {
     const int i = blockIdx.x;
     const int gl = threadIdx.x;
     if (i >= n) return;
@@ -381,6 +382,10 @@ __global__ __launch_bounds__(64) void k_initial_conditions(const double* r0, con
         omega0[i] = om;
         pphi0[i] = L;
         pr0[i] = c;
+        if (cond) {   // the integrator's initial state [4][n]: r, phi = 0, pr, pphi (pr and pphi rows are pr0 and pphi0)
+            cond[i] = r;
+            cond[(size_t)n + i] = 0.0;
+        }
     }
 }
 
@@ -803,8 +808,21 @@ int eobhip_initial_conditions(const double* r0, const void* bindata, const void*
     EOBHIP_REQUIRE(n >= 0 && max_iter >= 1, "bad arguments");
     if (n == 0) return EOBHIP_OK;
     hipLaunchKernelGGL(k_initial_conditions, dim3(n), dim3(64), 0, as_stream(stream), r0,
-                       static_cast<const BinData*>(bindata), static_cast<const ModeCoef*>(modes), n, pphi0, pr0, omega0, max_iter, err);
+                       static_cast<const BinData*>(bindata), static_cast<const ModeCoef*>(modes), n, pphi0, pr0, omega0, max_iter, err,
+                       nullptr);
     return check_launch("initial_conditions");
+}
+
+int eobhip_initial_state(const double* r0, const void* bindata, const void* modes, int n, double* cond, double* omega0, int max_iter,
+                         double err, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && max_iter >= 1, "bad arguments");
+    EOBHIP_REQUIRE(cond != nullptr && omega0 != nullptr, "null output");
+    if (n == 0) return EOBHIP_OK;
+    const size_t nn = (size_t)n;
+    hipLaunchKernelGGL(k_initial_conditions, dim3(n), dim3(64), 0, as_stream(stream), r0,
+                       static_cast<const BinData*>(bindata), static_cast<const ModeCoef*>(modes), n, cond + 3 * nn, cond + 2 * nn, omega0,
+                       max_iter, err, cond);
+    return check_launch("initial_state");
 }
 
 size_t eobhip_integrate_scratch_bytes(int n) {

@@ -111,14 +111,13 @@ class EOBBatch:
             r0 = r0.to(device=self.device, dtype=torch.float64).reshape(self.n)
         else:
             r0 = torch.as_tensor(np.broadcast_to(np.asarray(r0, dtype=np.float64), (self.n,)).copy(), device=self.device)
-        pphi0 = torch.empty(self.n, dtype=torch.float64, device=self.device)
-        pr0 = torch.empty_like(pphi0)
-        om0 = torch.empty_like(pphi0)
-        call("eobhip_initial_conditions", r0.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, pphi0.data_ptr(),
-             pr0.data_ptr(), om0.data_ptr(), int(max_iter), float(err), current_stream())
-        self.cond = torch.stack([r0, torch.zeros_like(r0), pr0, pphi0]).contiguous()
-        self.add[4] = om0
-        return pphi0, pr0, om0
+        # one launch writes the whole initial state and omega0 (additional-args row 4);
+        # every host launch before the integrator delays the pass
+        cond = torch.empty((4, self.n), dtype=torch.float64, device=self.device)
+        call("eobhip_initial_state", r0.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, cond.data_ptr(),
+             self.add[4].data_ptr(), int(max_iter), float(err), current_stream())
+        self.cond = cond
+        return cond[3], cond[2], self.add[4]   # pphi0, pr0, omega0 (views of the batch's state)
 
     def set_initial_state(self, cond):
         self.cond = torch.as_tensor(np.ascontiguousarray(cond, dtype=np.float64), device=self.device).reshape(4, self.n).contiguous()

@@ -134,6 +134,12 @@ int eobhip_bin_setup(const double* additionalArgs, int n, int use_hm, void* bind
  * reference's IC_diss false position on [-3e-2, 0] (src/EOB.cu:2666-2834). */
 int eobhip_initial_conditions(const double* r0, const void* bindata, const void* modes, int n, double* pphi0, double* pr0,
                               double* omega0, int max_iter, double err, void* stream);
+/* eobhip_initial_conditions written as the integrator's initial state:
+ * cond [4][n] = (r0, 0, pr0, pphi0) and omega0[n], in one launch.  No
+ * reference counterpart (the reference builds condBound from the three
+ * arrays, eob/seobnrv4phm.py:2086); a pass starts with it. */
+int eobhip_initial_state(const double* r0, const void* bindata, const void* modes, int n, double* cond, double* omega0, int max_iter,
+                         double err, void* stream);
 
 /* Integrate n binaries.  mode 0: one persistent work-stealing kernel (groups of `group`
  * lanes pull binaries from an atomic queue); mode 1: one kernel launch per attempted
