@@ -560,7 +560,9 @@ def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, po
     group first; at most ``nslot`` groups are in flight and a slot is given
     to a new group only once its previous group's second half is queued; the
     host never waits for a group's sizes while binaries may still finish.
-    Polls sleep ``poll_s`` while more than ``spin_below`` binaries remain.
+    While more than ``spin_below`` binaries remain, polls sleep ``poll_s`` and
+    ready second halves go before new groups; for the last ones, new groups
+    go first.
     Raises if ``integrator_ended()`` while flags are missing.  Returns the
     groups (index arrays into done) in launch order."""
     n = done.size
@@ -569,7 +571,11 @@ def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, po
     pending = collections.deque()   # (handle, slot) of groups whose second half is not queued yet, oldest first
     groups = []
     while True:
-        if pending and is_ready(pending[0][0]):
+        # while many binaries remain, queue ready second halves first; for the
+        # last few, start a newly finished group first (its first half then runs
+        # beside the previous group's second half instead of after its launches)
+        last_few = left <= spin_below
+        if pending and not last_few and is_ready(pending[0][0]):
             complete(pending.popleft()[0])
             continue
         if len(pending) < nslot:
@@ -582,6 +588,9 @@ def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, po
                 slot = next(i for i in range(nslot) if i not in busy)
                 pending.append((launch(new, slot), slot))
                 continue
+        if pending and last_few and is_ready(pending[0][0]):
+            complete(pending.popleft()[0])
+            continue
         if left == 0:
             if not pending:
                 break
