@@ -139,7 +139,7 @@ __host__ __device__ inline long long spline_scratch_per_interp(int max_step) {
 }
 // rows whose loads a thread issues together in the serial sweeps: the sweeps
 // would otherwise wait one memory round trip per row
-constexpr int kSweepBatch = 8;
+constexpr int kSweepBatch = 16;
 
 __global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const double* __restrict__ dense, const double* __restrict__ knots,
                                                                      const int* __restrict__ num_steps, int n, int max_step,
