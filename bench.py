@@ -187,26 +187,45 @@ def time_chunked(pipe, fractions, steps, warmup, group=64, block=64):
 def time_streamed(pipe, steps, warmup, group=64, block=64, grid_groups=0):
     """The same pass as Pipeline.run (support-length likelihood) as a StreamedPass:
     one integrator launch that flags each finished binary, knots, splines and
-    likelihood queued per group of finished binaries; returns (wall per pass,
-    log L, trajectory time, downstream groups of the last pass)."""
+    likelihood queued per group of finished binaries.  Every timed step
+    evaluates a new batch of binaries drawn from the same distribution (seeds
+    1000 + k): ``set_params`` orders them on the host and copies them to the
+    device, and the pass builds their additional args, tables and initial
+    state there.  Returns (wall per pass, log L of the batch the one-batch pass
+    ran, trajectory time, downstream groups of the last pass, host ingestion
+    time per pass in ms, attempted steps per pass)."""
     torch, P = pipe.torch, pipe.P
+    from eobgpu_amd import configs
     p = pipe.params
     sp = P.StreamedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], pipe.lik, pipe.M, pipe.dt, amp=pipe.amp,
                         max_step=pipe.b.max_step, rtol=pipe.rtol)
-    for _ in range(warmup):
+    sp.run(group=group, block=block, grid_groups=grid_groups)
+    torch.cuda.synchronize()
+    logl_same = sp.out.cpu().numpy().copy()
+    props = [configs.uneven_batch(n=pipe.n, seed=1000 + k) for k in range(warmup + steps)]
+
+    def ingest(q):
+        sp.set_params(q["m1"], q["m2"], q["chi1"], q["chi2"], q["r0"], amp=pipe.amp)
+
+    for k in range(warmup):
+        ingest(props[k])
         sp.run(group=group, block=block, grid_groups=grid_groups)
     torch.cuda.synchronize()
-    t_traj = 0.0
+    t_traj = t_ing = 0.0
+    attempts = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         ev = {"start": torch.cuda.Event(enable_timing=True), "traj_end_longest": torch.cuda.Event(enable_timing=True)}
         ev["start"].record()
+        ti = time.perf_counter()
+        ingest(props[warmup + k])
+        t_ing += time.perf_counter() - ti
         sp.run(group=group, block=block, grid_groups=grid_groups, events=ev)
         torch.cuda.synchronize()
         t_traj += ev["start"].elapsed_time(ev["traj_end_longest"])
-    torch.cuda.synchronize()
+        attempts.append(int(sp.batch.attempts.sum().item()))
     wall = (time.perf_counter() - t0) / steps
-    return wall, sp.out.cpu().numpy(), t_traj / steps, len(sp.groups)
+    return wall, logl_same, t_traj / steps, len(sp.groups), t_ing / steps * 1e3, attempts
 
 
 def cpu_baseline(pipe, nb):
@@ -219,31 +238,9 @@ def cpu_baseline(pipe, nb):
     p = pipe.params
     idx = np.arange(nb)
     data = pipe.data.cpu().numpy()
-    f = np.arange(pipe.F) * pipe.df
-    w = np.where(f > 0, 1.0 / pipe.P.analytic_psd(f), 0.0)
-    add = O.make_additional_args(p["m1"][idx], p["m2"][idx], p["chi1"][idx], p["chi2"][idx])
-    co = O.fac_coeffs(add)
     t0 = time.perf_counter()
-    pphi0, pr0, om0, add = O.initial_conditions(p["r0"][idx], add, co)
-    cond = np.array([p["r0"][idx], np.zeros(nb), pr0, pphi0])
-    t, traj, ns, si = O.run_trajectory(cond, add, co, max_step=pipe.b.max_step, rtol=pipe.rtol, atol=pipe.rtol)
-    h22 = O.get_hlm22(traj, ns, add, co)
-    logl = np.empty(nb)
-    for i in range(nb):
-        L = int(ns[i])
-        tk = t[i, :L]
-        phi = traj[i, 1, :L]
-        s22 = h22[i, :L] * np.exp(2j * phi)
-        y = np.stack([phi, s22.real, s22.imag], axis=1)              # [len][3 interpolants]
-        B, c1, c2, c3 = O.interpolate_TD(tk[:, None].copy(), y, np.array([L], dtype=np.int32), 1, 3)
-        dT = pipe.dt / (pipe.M[i] * pipe.P.MTSUN_SI)
-        nsamp = min(pipe.N, int((tk[-1] - tk[0]) / dT) + 1)
-        tj = tk[0] + np.arange(nsamp) * dT
-        ev = [O.spline_eval(tj, tk, y[:, q].copy(), c1[:, q].copy(), c2[:, q].copy(), c3[:, q].copy(), L) for q in range(3)]
-        h = np.zeros(pipe.N)
-        h[:nsamp] = pipe.amp[i] * (ev[1] * np.cos(2 * ev[0]) + ev[2] * np.sin(2 * ev[0]))
-        ht = np.fft.rfft(h)
-        logl[i] = -0.5 * 4 * pipe.df * np.sum(np.abs(data - ht) ** 2 * w)
+    logl, _, _ = O.pipeline_logl(p["m1"][idx], p["m2"][idx], p["chi1"][idx], p["chi2"][idx], p["r0"][idx], data, pipe.dt, pipe.N,
+                                 amp=pipe.amp[idx], max_step=pipe.b.max_step, rtol=pipe.rtol, atol=pipe.rtol)
     el = time.perf_counter() - t0
     return nb / el, el, logl
 
@@ -474,8 +471,11 @@ def main():
             headline_wall = wall_c
         print(f"[bench] chunked pass: {json.dumps(chunked)}", file=sys.stderr, flush=True)
     if args.pass_kind == "streamed":
-        wall_s, logl_s, traj_s, ngroups = time_streamed(pipe, args.steps, args.warmup, group=group, block=block, grid_groups=grid)
+        wall_s, logl_s, traj_s, ngroups, ingest_ms, att_s = time_streamed(pipe, args.steps, args.warmup, group=group, block=block,
+                                                                         grid_groups=grid)
         streamed = {"ms_per_step": wall_s * 1e3, "trajectory_ms": traj_s, "downstream_groups": ngroups,
+                    "host_ingest_ms": ingest_ms, "fresh_batches": True,
+                    "attempted_steps_per_pass": {"min": min(att_s), "max": max(att_s), "mean": float(np.mean(att_s))},
                     "one_batch_ms_per_step": wall * 1e3,
                     "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_s - logl_support) / np.abs(logl_support)))}
         headline_wall = wall_s

@@ -47,6 +47,9 @@ _SIGS = {
     "eobhip_bindata_bytes": [],
     "eobhip_modes_bytes": [],
     "eobhip_bin_setup": [c_vp, c_int, c_int, c_vp, c_vp, c_vp],
+    "eobhip_params_setup": [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_vp, c_vp,
+                            c_vp, c_vp],
+    "eobhip_newton_prefix_constants": [c_vp],
     "eobhip_initial_conditions": [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_int, c_double, c_vp],
     "eobhip_initial_state": [c_vp, c_vp, c_vp, c_int, c_vp, c_vp, c_int, c_double, c_vp],
     "eobhip_integrate_scratch_bytes": [c_int],
@@ -56,6 +59,9 @@ _SIGS = {
     "eobhip_integrate_signal": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
                                 c_int, c_int, c_vp, c_vp, c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                 c_vp, c_vp],
+    "eobhip_integrate_batch_coupled": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_double, c_double, c_double, c_double,
+                                       c_int, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll_p,
+                                       c_vp],
     "eobhip_hlm22_knots": [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp],
 }
 _RESTYPES = {

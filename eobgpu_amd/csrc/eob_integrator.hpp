@@ -105,7 +105,14 @@ __device__ __forceinline__ double* dense_row(const IntegParams& P, int b, int ro
 
 template <int G>
 __device__ __forceinline__ void write_row(const IntegParams& P, const GState<G>& s, int gl) {
-    double* row = dense_row(P, s.b, s.step_num);
+    // rows count from the end for a negative step number (numpy indexing, reached
+    // by a restart from a negative row); a restart from row max_step - 1 would
+    // store past the last row (the reference raises IndexError there), and such
+    // rows are dropped
+    int rr = s.step_num;
+    if (rr < 0) rr += P.max_step;
+    if (rr < 0 || rr >= P.max_step) return;
+    double* row = dense_row(P, s.b, rr);
 #pragma unroll
     for (int q = 0; q < GState<G>::NC; ++q) {
         const int c = comp_of<G>(gl, q);
@@ -180,7 +187,8 @@ __device__ __forceinline__ void restart_pass2(const IntegParams& P, GState<G>& s
     // argmin over rows 0..step_num (unwritten rows hold t = 0 and tie with row 0)
     double best_v = 1e308;
     int best_i = 0x7fffffff;
-    for (int row = gl; row <= s.step_num; row += G) {
+    const int last_row = (s.step_num < P.max_step - 1) ? s.step_num : P.max_step - 1;
+    for (int row = gl; row <= last_row; row += G) {
         const double d = fabs(coherent_load(dense_row(P, s.b, row)) - td);
         if (d < best_v) { best_v = d; best_i = row; }
     }
@@ -192,7 +200,7 @@ __device__ __forceinline__ void restart_pass2(const IntegParams& P, GState<G>& s
             if (ov < best_v || (ov == best_v && oi < best_i)) { best_v = ov; best_i = oi; }
         }
     }
-    if (best_i > s.step_num) best_i = 0;   // every |t - t_desired| was NaN
+    if (best_i > last_row) best_i = 0;   // every |t - t_desired| was NaN
     int row = best_i - 1;
     bool zero_state = false;
     if (row < 0) {

@@ -240,6 +240,11 @@ __global__ __launch_bounds__(kBlock) void k_error(double* err, const double* sol
                                                   const double* k2, const double* k3, const double* k6, const double* k7,
                                                   const double* k8, const double* k9, const double* k10, int n, int nargs,
                                                   double rtol, double atol) {
+    // The error estimate is a sum of stages that cancels by many orders of
+    // magnitude, so its value follows the rounding of every product: without
+    // contraction into fused multiply-adds the sums round as the reference's
+    // own host build of src/DOPR853.cu:586-616 rounds them.
+#pragma clang fp contract(off)
     for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
         double et = 0.0, e2 = 0.0;
         const double hh = h[e];
@@ -296,9 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_controller(unsigned char* flag, cons
 // straight into the binary's BinData.  With one binary per lane the 35 modes
 // ran in turn; this kernel starts every pass, so its latency is on the pass's
 // critical path.
-__global__ __launch_bounds__(64) void k_bin_setup(const double* add, int n, int use_hm, BinData* bd, ModeCoef* modes) {
-    const int i = blockIdx.x, lane = threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void bin_setup_one(const double* add, int n, int i, int lane, int use_hm, BinData* bd, ModeCoef* modes) {
     const double m1 = add[i], m2 = add[n + i], c1 = add[2 * n + i], c2 = add[3 * n + i];
     const double nu = nu_of(m1, m2);
     double ef = nu;
@@ -327,6 +330,107 @@ __global__ __launch_bounds__(64) void k_bin_setup(const double* add, int n, int 
         flux_setup(mc, d.w, m1, m2, eta, a, chiS, chiA, pf22, add[9 * n + i], use_hm, true);
         piece_setup(d.pc, d.h);
     }
+}
+
+__global__ __launch_bounds__(64) void k_bin_setup(const double* add, int n, int use_hm, BinData* bd, ModeCoef* modes) {
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    bin_setup_one(add, n, i, threadIdx.x, use_hm, bd, modes);
+}
+
+// The parameter-independent part of the 35 Newtonian multipole prefixes
+// (Damour, Iyer & Nagar 2008 Eqs. 5-7; EOBComputeNewtonMultipolePrefixes,
+// eob/seobnrv4phm.py:1703-1776): n_lm = (i m)^l 8 pi / (2l+1)!! sqrt((l+1)(l+2) /
+// (l(l-1))) for even l+m, -(i m)^l i 16 pi / (2l+1)!! sqrt(...) for odd, in
+// the order of operations of eobgpu_amd.pipeline.newton_prefixes so that the
+// device's prefixes equal the host's bit for bit.
+struct PrefixConsts {
+    double re[kNumModes], im[kNumModes];
+};
+
+PrefixConsts make_prefix_consts() {
+    PrefixConsts c{};
+    const double pi = 3.141592653589793;
+    int q = 0;
+    for (int l = 2; l <= 8; ++l) {
+        double dfac = 1.0;
+        for (int k = 2 * l + 1; k > 1; k -= 2) dfac *= k;
+        for (int m = 1; m <= l; ++m, ++q) {
+            double ml = 1.0;
+            for (int k = 0; k < l; ++k) ml *= m;
+            // (i m)^l: m^l times i^l
+            double nre = 0.0, nim = 0.0;
+            switch (l & 3) {
+                case 0: nre = ml; break;
+                case 1: nim = ml; break;
+                case 2: nre = -ml; break;
+                default: nim = -ml; break;
+            }
+            if (((l + m) & 1) == 0) {
+                const double a = (8.0 * pi) / dfac;
+                const double b = std::sqrt((double)((l + 1) * (l + 2)) / (double)(l * (l - 1)));
+                c.re[q] = (nre * a) * b;
+                c.im[q] = (nim * a) * b;
+            } else {
+                const double m2v = ((2.0 * l + 1) * (l + 2) * (l * l - m * m)) / ((2.0 * l - 1) * (l + 1) * l * (l - 1));
+                const double cim = (16.0 * pi) / dfac;
+                const double b = std::sqrt(m2v);
+                // -(i m)^l times (0 + i cim)
+                c.re[q] = (nim * cim) * b;
+                c.im[q] = (-(nre * cim)) * b;
+            }
+        }
+    }
+    return c;
+}
+
+// x^k for k >= 1 by repeated multiplication (the host's order)
+__device__ __forceinline__ double ipow_seq(double x, int k) {
+    double p = x;
+    for (int j = 1; j < k; ++j) p = p * x;
+    return p;
+}
+
+// additionalArgs [80][n] from the binaries' masses and spins, as
+// SEOBNRv4PHM.setup_class builds it on the host (eob/seobnrv4phm.py:1922-1964;
+// masses scaled to M = 1, omega0 = 0), then the tables of k_bin_setup, one
+// wavefront per binary: lane q < 35 writes the prefix of mode q.
+__global__ __launch_bounds__(64) void k_params_setup(const double* m1in, const double* m2in, const double* chi1, const double* chi2,
+                                                     int n, int use_hm, double K, double d5, double dSO, double dSS, double h22c,
+                                                     PrefixConsts pc, double* add, BinData* bd, ModeCoef* modes) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x, lane = threadIdx.x;
+    if (i >= n) return;
+    const double Mt = m1in[i] + m2in[i];
+    const double m1 = m1in[i] / Mt, m2 = m2in[i] / Mt;
+    if (lane == 0) {
+        add[i] = m1;
+        add[n + i] = m2;
+        add[2 * n + i] = chi1[i];
+        add[3 * n + i] = chi2[i];
+        add[4 * n + i] = 0.0;
+        add[5 * n + i] = K;
+        add[6 * n + i] = d5;
+        add[7 * n + i] = dSO;
+        add[8 * n + i] = dSS;
+        add[9 * n + i] = h22c;
+    }
+    if (lane < kNumModes) {
+        int l, m;
+        mode_lm(lane, l, m);
+        const int eps = (l + m) & 1;
+        const double M = m1 + m2;
+        const double x1 = m1 / M, x2 = m2 / M;
+        const double eta = (m1 * m2) / (M * M);
+        const double sign = (m & 1) ? -1.0 : 1.0;
+        double cdeg;
+        if (m1 != m2 || sign == 1.0) cdeg = ipow_seq(x2, l + eps - 1) + sign * ipow_seq(x1, l + eps - 1);
+        else cdeg = (l == 2 || l == 3) ? -1.0 : ((l == 4 || l == 5) ? -0.5 : 0.0);
+        add[(10 + 2 * lane) * n + i] = (pc.re[lane] * eta) * cdeg;
+        add[(11 + 2 * lane) * n + i] = (pc.im[lane] * eta) * cdeg;
+    }
+    __syncthreads();   // the column is in memory for the whole workgroup
+    bin_setup_one(add, n, i, lane, use_hm, bd, modes);
 }
 
 // The dissipative initial condition (IC_diss, src/EOB.cu:2666): dr/dt from the
@@ -523,6 +627,283 @@ __global__ __launch_bounds__(kBlock) void k_plain_step(IntegParams P, PlainState
         const int slot = atomicAdd(n_out, 1);
         active_out[slot] = b;
     }
+}
+
+// ---------------------------------------------- the reference's batch semantics
+// The reference's StoppingCriterion couples the binaries of a batch
+// (eob/seobnrv4phm.py:82-150): it runs on the binaries accepted in this driver
+// iteration, index_update, in ascending binary order; its third check reads
+// the omega history old_omegas[step-4 .. step-1] in column `q`, the binary's
+// POSITION inside index_update, not its own column; and while any binary of
+// index_update has passed step 5 without another stop code, only those
+// binaries store their omega (at their own column).  A binary's stop code and
+// step count then depend on its batch mates.  The coupled driver reproduces
+// this: one launch attempts a step for every active binary, a one-block launch
+// applies the criterion to the accepted ones in order, and both passes of
+// run_trajectory are batch-synchronous (pass 2 starts for every binary after
+// the last binary's pass 1 ends, eob/seobnrv4phm.py:2086-2123), each with its
+// omega history zeroed (StoppingCriterion.setup).  Binaries that the
+// non-finite guard ends in pass 1 sit out pass 2, whose history columns then
+// number the remaining binaries in order.
+struct CoupledState {
+    double* k1;       // [4][n] k1 at the binary's current state (valid when k1_ok)
+    int* k1_ok;       // [n]
+    double* chk;      // [4][n] r, dr/dt, omega, dpr/dt at the state just accepted
+    int* acc;         // [n] 1 if the binary's last attempt was accepted
+    int* code;        // [n] the criterion's code for the accepted binaries of this iteration
+    int* col;         // [n] the binary's column in the pass's omega history
+    double* old_om;   // [max_step][n] omega history (column stride n)
+};
+
+constexpr int kCoupledBlock = 1024;
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_coupled_attempt(IntegParams P, PlainState S, CoupledState C, const int* active,
+                                                            int n_active) {
+    const int gi = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int gl = threadIdx.x % G;
+    if (gi >= n_active) return;
+    const int b = active[gi];
+    const int n = P.nbin;
+    GState<G> s;
+    load_state<G>(S, s, b, gl, n);
+    const BinData& bd = P.bd[b];
+    const ModeCoef* mc = P.mc + (size_t)b * kNumModes;
+    double k1[4];
+    if (C.k1_ok[b]) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) k1[c] = C.k1[(size_t)c * n + b];
+    } else {
+        double yf[4];
+        gather_state<G>(s, gl, yf);
+        group_rhs<G>(bd, mc, yf[0], yf[2], yf[3], gl, k1);
+        if constexpr (G == 64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) k1[c] = read_lane(k1[c], 0);   // lane 63 holds the pr = 0 branch
+        }
+    }
+    const bool fixed = (s.pass == 2) || P.fix_step;
+    attempt_step<G>(P, s, bd, mc, k1, gl, fixed);
+    if (s.just_acc) {
+        // the criterion's right-hand side at the accepted state, which is also
+        // the next attempt's k1 (eob/seobnrv4phm.py:94)
+        double yf[4], kn[4];
+        gather_state<G>(s, gl, yf);
+        group_rhs<G>(bd, mc, yf[0], yf[2], yf[3], gl, kn);
+        if constexpr (G == 64) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) kn[c] = read_lane(kn[c], 0);
+        }
+        if (gl == 0) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) C.k1[(size_t)c * n + b] = kn[c];
+            C.chk[b] = yf[0];
+            C.chk[(size_t)n + b] = kn[0];
+            C.chk[2 * (size_t)n + b] = kn[1];
+            C.chk[3 * (size_t)n + b] = kn[2];
+        }
+    } else if (gl == 0) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) C.k1[(size_t)c * n + b] = k1[c];
+    }
+    if (gl == 0) {
+        C.k1_ok[b] = 1;
+        C.acc[b] = s.just_acc;
+    }
+    store_state<G>(S, s, gl, n);
+}
+
+// block-wide exclusive prefix sum of v (one value per thread); *total = the sum
+__device__ __forceinline__ int block_exclusive_scan(int v, int* total) {
+    __shared__ int part[kCoupledBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    __syncthreads();
+    if (lane == 63) part[wave] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        const int p = part[w];
+        if (w < wave) before += p;
+        all += p;
+    }
+    *total = all;
+    return before + incl - v;
+}
+
+// The criterion and the end-of-iteration bookkeeping for the active binaries,
+// in one block.  Thread t takes a contiguous run of the active list (which is in
+// ascending binary order), so positions inside index_update follow from one scan.
+__global__ __launch_bounds__(kCoupledBlock) void k_coupled_check(IntegParams P, PlainState S, CoupledState C, const int* active_in,
+                                                                 int n_active, int* active_out, int* n_out) {
+    const int n = P.nbin;
+    const int per = (n_active + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int lo = min((int)threadIdx.x * per, n_active), hi = min(lo + per, n_active);
+    // codes 2-4 and which accepted binaries go on to the omega checks (inds_here)
+    int n_acc = 0, any = 0;
+    for (int j = lo; j < hi; ++j) {
+        const int b = active_in[j];
+        if (!C.acc[b]) continue;
+        ++n_acc;
+        const double r = C.chk[b], drdt = C.chk[(size_t)n + b], dprdt = C.chk[3 * (size_t)n + b];
+        int code = 0;
+        if (r < 1.0) code = 2;
+        else if (r < 6.0 && drdt > 0.0) code = 3;
+        else if (r < 6.0 && dprdt > 0.0) code = 4;
+        if (!P.use_stop) code = 0;
+        const int here = P.use_stop && code == 0 && S.step_num[b] > 5;
+        any |= here;
+        C.code[b] = code | (here << 8);
+    }
+    int n_acc_all;
+    int q = block_exclusive_scan(n_acc, &n_acc_all);
+    const int any_here = __syncthreads_or(any);
+    // third check against column q; every read of the history happens before any store
+    for (int j = lo; j < hi; ++j) {
+        const int b = active_in[j];
+        if (!C.acc[b]) continue;
+        const int cv = C.code[b];
+        if ((cv >> 8) && S.step_num[b] <= P.max_step) {
+            const int s = S.step_num[b];
+            const double r = C.chk[b], om = C.chk[2 * (size_t)n + b];
+            const double* h = C.old_om;
+            const bool first = r < 6.0;
+            const bool second = om < h[(size_t)(s - 1) * n + C.col[b]];
+            const double o0 = h[(size_t)(s - 4) * n + q], o1 = h[(size_t)(s - 3) * n + q], o2 = h[(size_t)(s - 2) * n + q],
+                         o3 = h[(size_t)(s - 1) * n + q];
+            const bool third = (o1 - o0 < 0.0) && (o2 - o1 < 0.0) && (o3 - o2 < 0.0);
+            if (first && second && third) C.code[b] = cv + 5;
+        }
+        ++q;
+    }
+    __syncthreads();
+    // stores, codes, and the end of pass for every active binary
+    int n_keep = 0;
+    for (int j = lo; j < hi; ++j) {
+        const int b = active_in[j];
+        int f = S.flags[b];
+        if (C.acc[b]) {
+            const int cv = C.code[b];
+            const int code = cv & 0xff;
+            const int s = S.step_num[b];
+            const int srow = (s < 0) ? s + P.max_step : s;   // numpy's negative rows
+            if (P.use_stop && (!any_here || (cv >> 8)) && srow >= 0 && srow < P.max_step)
+                C.old_om[(size_t)srow * n + C.col[b]] = C.chk[2 * (size_t)n + b];
+            S.code[b] = code;
+            f = (f & ~4) | ((code != 0) << 2);
+            S.flags[b] = f;
+        }
+        const int fail = (f >> 4) & 3;
+        const int step_num = S.step_num[b];
+        const bool term = (S.x[b] >= P.tmax) || (step_num >= P.max_step - 1) || ((f >> 2) & 1) || fail;
+        if (!term) { ++n_keep; continue; }
+        const int pass = S.pass[b];
+        if (fail) {
+            S.code[b] = -fail;
+            if (pass == 1) S.sn1[b] = step_num;
+        }
+        if (pass == 1 && P.two_pass && !fail) {
+            S.sn1[b] = step_num;
+            S.pass[b] = 3;   // pass 1 over; pass 2 starts with the batch's
+        } else {
+            P.num_steps[b] = step_num + 1;
+            P.stop_info[b] = S.code[b];
+            P.pass1_steps[b] = P.two_pass ? S.sn1[b] : step_num;
+            P.attempts[b] = S.att[b];
+            S.pass[b] = 0;
+        }
+    }
+    int n_keep_all;
+    int w = block_exclusive_scan(n_keep, &n_keep_all);
+    for (int j = lo; j < hi; ++j) {
+        const int b = active_in[j];
+        const int p = S.pass[b];
+        if (p == 1 || p == 2) active_out[w++] = b;
+    }
+    if (threadIdx.x == 0) *n_out = n_keep_all;
+}
+
+// Pass 2 of the coupled driver: the binaries whose pass 1 ended normally, in
+// ascending order, numbered 0.. as the columns of the pass's omega history.
+__global__ __launch_bounds__(kCoupledBlock) void k_coupled_pass2_list(IntegParams P, PlainState S, CoupledState C, int* active_out,
+                                                                      int* n_out) {
+    const int n = P.nbin;
+    const int per = (n + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int lo = min((int)threadIdx.x * per, n), hi = min(lo + per, n);
+    int cnt = 0;
+    for (int b = lo; b < hi; ++b) cnt += (S.pass[b] == 3);
+    int total;
+    int w = block_exclusive_scan(cnt, &total);
+    for (int b = lo; b < hi; ++b) {
+        if (S.pass[b] != 3) continue;
+        C.col[b] = w;
+        active_out[w++] = b;
+    }
+    if (threadIdx.x == 0) *n_out = total;
+}
+
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_coupled_restart(IntegParams P, PlainState S, CoupledState C, const int* active,
+                                                            int n_active) {
+    const int gi = (blockIdx.x * blockDim.x + threadIdx.x) / G;
+    const int gl = threadIdx.x % G;
+    if (gi >= n_active) return;
+    const int b = active[gi];
+    GState<G> s;
+    load_state<G>(S, s, b, gl, P.nbin);
+    restart_pass2<G>(P, s, gl);
+    if (gl == 0) C.k1_ok[b] = 0;
+    store_state<G>(S, s, gl, P.nbin);
+}
+
+__global__ void k_iota(int* a, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = i;
+}
+
+template <int G>
+int run_coupled(const IntegParams& P, PlainState S, CoupledState C, int* act0, int* act1, int* cnt, int* host_cnt, hipStream_t st,
+                long long* iters) {
+    const int gpb = kBlock / G;
+    const size_t hist_bytes = sizeof(double) * (size_t)P.max_step * P.nbin;
+    hipLaunchKernelGGL(k_plain_init<G>, dim3(blocks_for(P.nbin, gpb)), dim3(kBlock), 0, st, P, S, act0);
+    int rc = check_launch("k_plain_init");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(P.nbin, kBlock)), dim3(kBlock), 0, st, C.col, P.nbin);
+    if ((rc = check_launch("k_iota"))) return rc;
+    if ((rc = hip_status(hipMemsetAsync(C.k1_ok, 0, sizeof(int) * P.nbin, st), "memset k1_ok"))) return rc;
+    long long it = 0;
+    int n_active = P.nbin;
+    for (int pass = 1; pass <= (P.two_pass ? 2 : 1); ++pass) {
+        if (pass == 2) {
+            hipLaunchKernelGGL(k_coupled_pass2_list, dim3(1), dim3(kCoupledBlock), 0, st, P, S, C, act0, cnt);
+            if ((rc = check_launch("k_coupled_pass2_list"))) return rc;
+            if ((rc = hip_status(hipMemcpyAsync(host_cnt, cnt, sizeof(int), hipMemcpyDeviceToHost, st), "count copy"))) return rc;
+            if ((rc = hip_status(hipStreamSynchronize(st), "sync"))) return rc;
+            n_active = *host_cnt;
+            if (n_active == 0) break;
+            hipLaunchKernelGGL(k_coupled_restart<G>, dim3(blocks_for(n_active, gpb)), dim3(kBlock), 0, st, P, S, C, act0, n_active);
+            if ((rc = check_launch("k_coupled_restart"))) return rc;
+        }
+        if (P.use_stop && (rc = hip_status(hipMemsetAsync(C.old_om, 0, hist_bytes, st), "memset old_omegas"))) return rc;
+        while (n_active > 0) {
+            hipLaunchKernelGGL(k_coupled_attempt<G>, dim3(blocks_for(n_active, gpb)), dim3(kBlock), 0, st, P, S, C, act0, n_active);
+            if ((rc = check_launch("k_coupled_attempt"))) return rc;
+            hipLaunchKernelGGL(k_coupled_check, dim3(1), dim3(kCoupledBlock), 0, st, P, S, C, act0, n_active, act1, cnt);
+            if ((rc = check_launch("k_coupled_check"))) return rc;
+            if ((rc = hip_status(hipMemcpyAsync(host_cnt, cnt, sizeof(int), hipMemcpyDeviceToHost, st), "count copy"))) return rc;
+            if ((rc = hip_status(hipStreamSynchronize(st), "sync"))) return rc;
+            n_active = *host_cnt;
+            int* t = act0; act0 = act1; act1 = t;
+            ++it;
+        }
+    }
+    if (iters) *iters = it;
+    return EOBHIP_OK;
 }
 
 // (2,2) mode on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221).
@@ -803,6 +1184,26 @@ int eobhip_bin_setup(const double* add, int n, int use_hm, void* bindata, void* 
     return check_launch("bin_setup");
 }
 
+int eobhip_params_setup(const double* m1, const double* m2, const double* chi1, const double* chi2, int n, int use_hm, double K,
+                        double d5, double dSO, double dSS, double h22_calib, double* add, void* bindata, void* modes, void* stream) {
+    EOBHIP_REQUIRE(n >= 0, "n < 0");
+    if (n == 0) return EOBHIP_OK;
+    static const PrefixConsts pc = make_prefix_consts();
+    hipLaunchKernelGGL(k_params_setup, dim3(n), dim3(64), 0, as_stream(stream), m1, m2, chi1, chi2, n, use_hm, K, d5, dSO, dSS,
+                       h22_calib, pc, add, static_cast<BinData*>(bindata), static_cast<ModeCoef*>(modes));
+    return check_launch("params_setup");
+}
+
+int eobhip_newton_prefix_constants(double* out) {
+    EOBHIP_REQUIRE(out, "null output");
+    const PrefixConsts pc = make_prefix_consts();
+    for (int q = 0; q < kNumModes; ++q) {
+        out[2 * q] = pc.re[q];
+        out[2 * q + 1] = pc.im[q];
+    }
+    return EOBHIP_OK;
+}
+
 int eobhip_initial_conditions(const double* r0, const void* bindata, const void* modes, int n, double* pphi0, double* pr0,
                               double* omega0, int max_iter, double err, void* stream) {
     EOBHIP_REQUIRE(n >= 0 && max_iter >= 1, "bad arguments");
@@ -827,9 +1228,11 @@ int eobhip_initial_state(const double* r0, const void* bindata, const void* mode
 
 size_t eobhip_integrate_scratch_bytes(int n) {
     const size_t nn = (size_t)(n > 0 ? n : 1);
-    // queue/count ints + plain-mode state: 6 int arrays, 4 + 4 + 4 double rows, att, 2 active lists
-    // 16 sub-arrays, each padded to a 256-byte boundary
-    return 256 + nn * (6 * sizeof(int) + 12 * sizeof(double) + sizeof(unsigned long long) + 2 * sizeof(int)) + 16 * 256;
+    // queue/count ints + plain-mode state: 6 int arrays, 4 + 4 + 4 double rows, att, 2 active lists;
+    // the coupled driver's k1 and criterion rows (4 + 4 doubles) and 4 int arrays;
+    // 22 sub-arrays, each padded to a 256-byte boundary
+    return 256 + nn * (6 * sizeof(int) + 12 * sizeof(double) + sizeof(unsigned long long) + 2 * sizeof(int)) +
+           nn * (8 * sizeof(double) + 4 * sizeof(int)) + 22 * 256;
 }
 
 }  // extern "C"
@@ -840,11 +1243,12 @@ int integrate_impl(const double* condBound, const void* bindata, const void* mod
                    double hInit, double step_back, double fine_step, double tmax, int two_pass, int fix_step, const double* x0,
                    const int* step0, int use_stop, int mode, int group, int block, int grid_groups, double* dense, int* num_steps,
                    int* stop_info, int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out, int* done,
-                   void* stream) {
+                   double* old_om, void* stream) {
     EOBHIP_REQUIRE((x0 == nullptr) == (step0 == nullptr), "x0 and step0 go together");
     EOBHIP_REQUIRE(n >= 0, "n < 0");
     EOBHIP_REQUIRE(max_step >= 2, "max_step must be >= 2");
-    EOBHIP_REQUIRE(mode == 0 || mode == 1, "mode must be 0 (persistent) or 1 (plain)");
+    EOBHIP_REQUIRE(mode == 0 || mode == 1 || mode == 2, "mode must be 0 (persistent), 1 (plain) or 2 (coupled)");
+    EOBHIP_REQUIRE(mode != 2 || old_om != nullptr || !use_stop, "the coupled driver needs old_omegas");
     EOBHIP_REQUIRE(group == 1 || group == 4 || group == 8 || group == 16 || group == 32 || group == 64,
                    "group must be one of 1, 4, 8, 16, 32, 64");
     if (block <= 0) block = kBlock;
@@ -912,6 +1316,16 @@ int integrate_impl(const double* condBound, const void* bindata, const void* mod
     S.att = static_cast<unsigned long long*>(take(nn * sizeof(unsigned long long)));
     int* act0 = static_cast<int*>(take(nn * sizeof(int)));
     int* act1 = static_cast<int*>(take(nn * sizeof(int)));
+    CoupledState C{};
+    if (mode == 2) {
+        C.k1 = static_cast<double*>(take(4 * nn * sizeof(double)));
+        C.chk = static_cast<double*>(take(4 * nn * sizeof(double)));
+        C.k1_ok = static_cast<int*>(take(nn * sizeof(int)));
+        C.acc = static_cast<int*>(take(nn * sizeof(int)));
+        C.code = static_cast<int*>(take(nn * sizeof(int)));
+        C.col = static_cast<int*>(take(nn * sizeof(int)));
+        C.old_om = old_om;
+    }
     if (off > eobhip_integrate_scratch_bytes(n)) {
         set_error("internal: scratch layout exceeds eobhip_integrate_scratch_bytes");
         return EOBHIP_EINVAL;
@@ -919,6 +1333,18 @@ int integrate_impl(const double* condBound, const void* bindata, const void* mod
     int* host_cnt = nullptr;
     rc = hip_status(hipHostMalloc(&host_cnt, sizeof(int)), "hipHostMalloc");
     if (rc) return rc;
+    if (mode == 2) {
+        switch (group) {
+            case 1: rc = run_coupled<1>(P, S, C, act0, act1, queue + 1, host_cnt, st, iterations_out); break;
+            case 4: rc = run_coupled<4>(P, S, C, act0, act1, queue + 1, host_cnt, st, iterations_out); break;
+            case 8: rc = run_coupled<8>(P, S, C, act0, act1, queue + 1, host_cnt, st, iterations_out); break;
+            case 16: rc = run_coupled<16>(P, S, C, act0, act1, queue + 1, host_cnt, st, iterations_out); break;
+            case 32: rc = run_coupled<32>(P, S, C, act0, act1, queue + 1, host_cnt, st, iterations_out); break;
+            default: rc = run_coupled<64>(P, S, C, act0, act1, queue + 1, host_cnt, st, iterations_out); break;
+        }
+        hipHostFree(host_cnt);
+        return rc;
+    }
     switch (group) {
         case 1: rc = run_plain<1>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
         case 4: rc = run_plain<4>(P, S, act0, act1, queue + 1, host_cnt, block, st, iterations_out); break;
@@ -941,7 +1367,7 @@ int eobhip_integrate(const double* condBound, const void* bindata, const void* m
                      int* pass1_steps, unsigned long long* attempts, void* scratch, long long* iterations_out, void* stream) {
     return integrate_impl(condBound, bindata, modes, n, max_step, rtol, atol, hInit, step_back, fine_step, tmax, two_pass, fix_step, x0,
                           step0, use_stop, mode, group, block, grid_groups, dense, num_steps, stop_info, pass1_steps, attempts, scratch,
-                          iterations_out, nullptr, stream);
+                          iterations_out, nullptr, nullptr, stream);
 }
 
 int eobhip_integrate_signal(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol,
@@ -960,7 +1386,17 @@ int eobhip_integrate_signal(const double* condBound, const void* bindata, const 
     }
     return integrate_impl(condBound, bindata, modes, n, max_step, rtol, atol, hInit, step_back, fine_step, tmax, two_pass, fix_step, x0,
                           step0, use_stop, 0, group, block, grid_groups, dense, num_steps, stop_info, pass1_steps, attempts, scratch,
-                          nullptr, static_cast<int*>(done_dev), stream);
+                          nullptr, static_cast<int*>(done_dev), nullptr, stream);
+}
+
+int eobhip_integrate_batch_coupled(const double* condBound, const void* bindata, const void* modes, int n, int max_step, double rtol,
+                                   double atol, double hInit, double step_back, double fine_step, double tmax, int two_pass,
+                                   int fix_step, const double* x0, const int* step0, int use_stop, int group, double* dense,
+                                   int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts, double* old_omegas,
+                                   void* scratch, long long* iterations_out, void* stream) {
+    return integrate_impl(condBound, bindata, modes, n, max_step, rtol, atol, hInit, step_back, fine_step, tmax, two_pass, fix_step, x0,
+                          step0, use_stop, 2, group, kBlock, 0, dense, num_steps, stop_info, pass1_steps, attempts, scratch,
+                          iterations_out, nullptr, old_omegas, stream);
 }
 
 int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n, int max_step,

@@ -22,10 +22,17 @@ Semantics kept from the reference:
     stopping criterion returns a non-zero code (:761);
   * stop codes 2/3/4/5 (eob/seobnrv4phm.py:113-133);
   * a restart row index of -1 addresses the last row, as numpy indexing does.
-Each binary is integrated as a batch of one: the reference's criterion
-couples lanes through its omega bookkeeping (``still_have`` indexes the
-updated subset, eob/seobnrv4phm.py:121-140), which makes a binary's result
-depend on its batch mates; the kernel evaluates the criterion per binary.
+
+The reference's criterion couples the binaries of a batch: its third check
+reads the omega history at a binary's position among the binaries accepted in
+the iteration, and while any binary is past step 5 only those store their
+omega (eob/seobnrv4phm.py:121-140), so a binary's stop code and step count
+depend on its batch mates.  ``DOPR853(batch_coupled=True)`` (the default when a
+stopping criterion is given) reproduces that with the coupled driver
+(``eobhip_integrate_batch_coupled``: one launch per attempted step plus a
+one-block criterion launch); ``batch_coupled=False`` integrates every binary as
+a batch of one in the persistent kernel, whose results do not depend on the
+batch (the batched pipeline and the server use that form).
 """
 from __future__ import annotations
 
@@ -52,12 +59,16 @@ class StoppingCriterion:
     """eob/seobnrv4phm.py:61-150.
 
     Passing an instance to ``DOPR853`` switches the EOB stop codes on inside the
-    integrator kernel, where they are evaluated on the k1 stage of the step
+    integrator kernels, where they are evaluated on the k1 stage of the step
     after each accepted one (that stage is f(x, y_new), the same right-hand
-    side the reference evaluates again for the criterion, :94).  ``__call__``
-    is the same decision as a standalone batched call for callers that drive
+    side the reference evaluates again for the criterion, :94); the coupled
+    driver keeps its omega history in ``old_omegas``.  ``__call__`` is the
+    reference's decision as a standalone batched call for callers that drive
     their own stepping: it evaluates the right-hand side through
-    ``eob_c_class.ODE_Ham_align_AD`` and applies codes 2-5 per binary.
+    ``eob_c_class.ODE_Ham_align_AD`` and applies codes 2-5 with the
+    reference's indexing of the omega history (third check at the position
+    inside ``index_update``; omegas stored only for the binaries past step 5
+    while any is, eob/seobnrv4phm.py:113-140).
     """
 
     def __init__(self, eob_c_class, max_step, use_gpu=False, read_out_to_cpu=False):
@@ -97,12 +108,17 @@ class StoppingCriterion:
         sn = sn_all[idx]
         here = (stop == 0) & (sn > 5)
         if bool(here.any()):
-            rows = sn.clamp(min=4)
-            hist = torch.stack([self.old_omegas[rows - k, idx] for k in (4, 3, 2, 1)])
-            falling = (hist[1:] - hist[:-1] < 0.0).all(dim=0)
-            c5 = here & (r < 6.0) & (omega < hist[3]) & falling
-            stop = torch.where(c5, 5, stop)
-        self.old_omegas[sn, idx] = omega
+            still = torch.nonzero(here).reshape(-1)          # positions inside index_update
+            s_st = sn[still]
+            second = omega[still] < self.old_omegas[s_st - 1, idx[still]]
+            rows = s_st[:, None] + torch.arange(-4, 0, device=dev)[None, :]
+            hist = self.old_omegas[rows, still[:, None].expand(-1, 4)]   # column = position, as the reference
+            third = (hist[:, 1:] - hist[:, :-1] < 0.0).all(dim=1)
+            c5 = (r[still] < 6.0) & second & third
+            stop[still] = stop[still] + 5 * c5.to(torch.int64)
+            self.old_omegas[s_st, idx[still]] = omega[still]
+        else:
+            self.old_omegas[sn, idx] = omega
         return stop
 
 
@@ -118,7 +134,7 @@ class DOPR853:
     """
 
     def __init__(self, ode, stopping_criterion=None, tmax=1e300, max_step=int(1e6), use_gpu=True, read_out_to_cpu=False,
-                 group=64, block=256):
+                 group=64, block=256, batch_coupled=True):
         if ode is not None and not isinstance(ode, SEOBNRv5Class):
             raise TypeError("DOPR853 integrates the EOB equations of motion only: pass a pyEOB.SEOBNRv5Class (or None)")
         if not use_gpu:
@@ -132,6 +148,7 @@ class DOPR853:
         self.read_out_to_cpu = bool(read_out_to_cpu)
         self.group = int(group)
         self.block = int(block)
+        self.batch_coupled = bool(batch_coupled)
         self.stop_info = None
         self.attempts = None
 
@@ -163,6 +180,9 @@ class DOPR853:
                 raise ValueError("step_num must have one entry per binary")
             if bool(((sn < -max_step) | (sn >= max_step)).any()):
                 raise IndexError("step_num out of range of denseOutput")
+            if bool((sn >= max_step - 1).any()):
+                # the reference's first accepted step would index denseOutput[max_step]
+                raise IndexError("step_num leaves no row of denseOutput for the next step")
             row = torch.where(sn < 0, sn + max_step, sn)
             ar = torch.arange(n, device=dev)
             x0 = dloc[row, ar].contiguous()
@@ -172,11 +192,23 @@ class DOPR853:
             batch.dense[:, :, 1:] = dout.permute(2, 0, 1)
             step0 = sn.to(torch.int32).contiguous()
         batch.cond = cond
-        # the kernel keeps the criterion's omega history (last four values) in
-        # registers, so the [max_step][n] history array is not allocated here
         use_stop = self.stopping_criterion is not None
-        batch.integrate(hInit=float(hInit), two_pass=False, fix_step=bool(fix_step), tmax=self.tmax, x0=x0, step0=step0,
-                        use_stop=use_stop, group=self.group, block=self.block)
+        sc = self.stopping_criterion
+        if use_stop and hasattr(sc, "setup"):
+            sc.setup(n)
+        if use_stop and self.batch_coupled:
+            # the reference's batch semantics; the criterion's history array is the driver's
+            om = getattr(sc, "old_omegas", None)
+            if isinstance(om, torch.Tensor) and om.is_cuda and om.dtype == torch.float64 and tuple(om.shape) == (max_step, n) \
+                    and om.is_contiguous():
+                batch.old_omegas = om
+            batch.integrate(hInit=float(hInit), two_pass=False, fix_step=bool(fix_step), tmax=self.tmax, x0=x0, step0=step0,
+                            use_stop=True, mode="coupled", group=self.group)
+        else:
+            # each binary as a batch of one: the persistent kernel keeps the last
+            # four omegas in registers
+            batch.integrate(hInit=float(hInit), two_pass=False, fix_step=bool(fix_step), tmax=self.tmax, x0=x0, step0=step0,
+                            use_stop=use_stop, group=self.group, block=self.block)
         sync()
         loc = batch.dense[:, :, 0].T.contiguous()
         traj = batch.dense[:, :, 1:].permute(1, 2, 0).contiguous()

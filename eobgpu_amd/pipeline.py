@@ -33,34 +33,60 @@ def _dbl_factorial(n):
     return out
 
 
+def _ipow_seq(x, k):
+    """x^k (k >= 1) by repeated multiplication, the order the device uses."""
+    p = x
+    for _ in range(k - 1):
+        p = p * x
+    return p
+
+
+def newton_prefix_constants():
+    """The parameter-independent factors n_lm of the 35 prefixes (complex [35]):
+    (i m)^l 8 pi / (2l+1)!! sqrt((l+1)(l+2) / (l(l-1))) for even l + m and
+    -(i m)^l i 16 pi / (2l+1)!! sqrt(m2v) for odd (Damour, Iyer & Nagar 2008
+    Eqs. 5-7)."""
+    out = np.zeros(35, dtype=np.complex128)
+    q = 0
+    for l in range(2, 9):
+        for m in range(1, l + 1):
+            df = _dbl_factorial(2 * l + 1)
+            n = (1j * m) ** l
+            if (l + m) % 2 == 0:
+                n = n * (8.0 * math.pi / df) * math.sqrt((l + 1) * (l + 2) / (l * (l - 1)))
+            else:
+                m2v = (2.0 * l + 1) * (l + 2) * (l * l - m * m) / ((2.0 * l - 1) * (l + 1) * l * (l - 1))
+                n = -n * (1j * 16.0 * math.pi / df) * math.sqrt(m2v)
+            out[q] = n
+            q += 1
+    return out
+
+
 def newton_prefixes(m1, m2):
     """Newtonian multipole prefixes for the 35 flux modes (l = 2..8, m = 1..l).
 
     Damour, Iyer & Nagar 2008 Eqs. 5-7, following EOBComputeNewtonMultipolePrefixes
-    (eob/seobnrv4phm.py:1703-1776).  Returns complex [35][n].
+    (eob/seobnrv4phm.py:1703-1776).  Returns complex [35][n].  The arithmetic is
+    ordered as the device's (``eobhip_params_setup``), which gives the same bits.
     """
     m1 = np.atleast_1d(np.asarray(m1, dtype=np.float64))
     m2 = np.atleast_1d(np.asarray(m2, dtype=np.float64))
     M = m1 + m2
     x1, x2 = m1 / M, m2 / M
     eta = m1 * m2 / (M * M)
+    nlm = newton_prefix_constants()
     out = np.zeros((35, m1.size), dtype=np.complex128)
     q = 0
     for l in range(2, 9):
         for m in range(1, l + 1):
             eps = (l + m) % 2
-            sign = 1 if m % 2 == 0 else -1
-            t1 = (m1 != m2) | (sign == 1)
-            cdeg = np.where(t1, x2 ** (l + eps - 1) + sign * x1 ** (l + eps - 1),
+            sign = 1.0 if m % 2 == 0 else -1.0
+            t1 = (m1 != m2) | (sign == 1.0)
+            k = l + eps - 1
+            cdeg = np.where(t1, _ipow_seq(x2, k) + sign * _ipow_seq(x1, k),
                             -1.0 if l in (2, 3) else (-0.5 if l in (4, 5) else 0.0))
-            df = _dbl_factorial(2 * l + 1)
-            n = (1j * m) ** l
-            if eps == 0:
-                n = n * (8.0 * math.pi / df) * math.sqrt((l + 1) * (l + 2) / (l * (l - 1)))
-            else:
-                m2v = (2.0 * l + 1) * (l + 2) * (l * l - m * m) / ((2.0 * l - 1) * (l + 1) * l * (l - 1))
-                n = -n * (1j * 16.0 * math.pi / df) * math.sqrt(m2v)
-            out[q] = n * eta * cdeg
+            out[q].real = (nlm[q].real * eta) * cdeg
+            out[q].imag = (nlm[q].imag * eta) * cdeg
             q += 1
     return out
 
@@ -89,14 +115,59 @@ class EOBBatch:
             add = additional_args.to(device=device, dtype=torch.float64).contiguous()
         else:
             add = torch.as_tensor(np.ascontiguousarray(additional_args, dtype=np.float64), device=device)
-        self.n = int(add.shape[1])
+        self._alloc(int(add.shape[1]), max_step, device)
+        self.add = add
+        self.use_hm = int(use_hm)
+        call("eobhip_bin_setup", add.data_ptr(), self.n, int(use_hm), self.bindata.data_ptr(), self.modes.data_ptr(), current_stream())
+
+    @classmethod
+    def from_params(cls, m1, m2, chi1, chi2, max_step=4096, device="cuda", use_hm=1):
+        """A batch whose additional args and tables are built on the device from
+        the masses and spins (``set_params``)."""
+        m1 = m1 if isinstance(m1, torch.Tensor) else np.atleast_1d(np.asarray(m1, dtype=np.float64))
+        b = cls.__new__(cls)
+        b._alloc(int(m1.shape[0]), max_step, device)
+        b.add = torch.empty((NADD, b.n), dtype=torch.float64, device=device)
+        b.use_hm = int(use_hm)
+        b.set_params(m1, m2, chi1, chi2)
+        return b
+
+    def set_params(self, m1, m2, chi1, chi2):
+        """New binaries for the same buffers: masses (any common unit) and aligned
+        spins [n], as device tensors or host values; additional args and tables are
+        built on the device in one launch (``eobhip_params_setup``, the device form
+        of SEOBNRv4PHM.setup_class, eob/seobnrv4phm.py:1922-1964).  Host values go
+        to the device in one copy through pinned memory."""
+        n = self.n
+        if all(isinstance(x, torch.Tensor) and x.device.type == "cuda" for x in (m1, m2, chi1, chi2)):
+            cols = [x.to(dtype=torch.float64).reshape(n).contiguous() for x in (m1, m2, chi1, chi2)]
+        else:
+            st = getattr(self, "_param_stage", None)
+            if st is None:
+                self._param_stage = st = torch.empty((4, n), dtype=torch.float64, pin_memory=True)
+                self._param_dev = torch.empty((4, n), dtype=torch.float64, device=self.device)
+            # the previous copy out of the staging rows must be done before they are rewritten
+            ev = getattr(self, "_param_copied", None)
+            if ev is not None:
+                ev.synchronize()
+            a = st.numpy()
+            for j, x in enumerate((m1, m2, chi1, chi2)):
+                a[j] = x.cpu().numpy() if isinstance(x, torch.Tensor) else np.broadcast_to(np.asarray(x, dtype=np.float64), (n,))
+            self._param_dev.copy_(st, non_blocking=True)
+            self._param_copied = torch.cuda.Event()
+            self._param_copied.record(torch.cuda.current_stream(self.device))
+            cols = [self._param_dev[j] for j in range(4)]
+        call("eobhip_params_setup", cols[0].data_ptr(), cols[1].data_ptr(), cols[2].data_ptr(), cols[3].data_ptr(), n, self.use_hm,
+             0.0, 0.0, 0.0, 0.0, 0.0, self.add.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), current_stream())
+        self._param_cols = cols   # keep device inputs alive while the launch is queued
+
+    def _alloc(self, n, max_step, device):
+        self.n = int(n)
         self.device = device
         self.max_step = int(max_step)
-        self.add = add
         L = lib()
         self.bindata = torch.empty(int(L.eobhip_bindata_bytes()) * self.n, dtype=torch.uint8, device=device)
         self.modes = torch.empty(int(L.eobhip_modes_bytes()) * self.n, dtype=torch.uint8, device=device)
-        call("eobhip_bin_setup", add.data_ptr(), self.n, int(use_hm), self.bindata.data_ptr(), self.modes.data_ptr(), current_stream())
         self.dense = torch.zeros((self.n, self.max_step, 5), dtype=torch.float64, device=device)
         self.num_steps = torch.zeros(self.n, dtype=torch.int32, device=device)
         self.stop_info = torch.zeros(self.n, dtype=torch.int32, device=device)
@@ -107,6 +178,9 @@ class EOBBatch:
 
     # ------------------------------------------------------------------ setup
     def initial_conditions(self, r0, max_iter=1000, err=1e-12):
+        """Circular start at r0 (device tensor or host values [n]); returns
+        (pphi0, pr0, omega0) as views of the batch's state that the next call
+        overwrites."""
         if isinstance(r0, torch.Tensor):   # a device tensor is used as is (no host round trip)
             r0 = r0.to(device=self.device, dtype=torch.float64).reshape(self.n)
         else:
@@ -117,7 +191,10 @@ class EOBBatch:
         call("eobhip_initial_state", r0.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n, cond.data_ptr(),
              self.add[4].data_ptr(), int(max_iter), float(err), current_stream())
         self.cond = cond
-        return cond[3], cond[2], self.add[4]   # pphi0, pr0, omega0 (views of the batch's state)
+        # views, not copies (a copy would cost a launch in every pass): pphi0 and
+        # pr0 are rows of the batch's initial state and omega0 is additional-args
+        # row 4, so the next call overwrites all three
+        return cond[3], cond[2], self.add[4]
 
     def set_initial_state(self, cond):
         self.cond = torch.as_tensor(np.ascontiguousarray(cond, dtype=np.float64), device=self.device).reshape(4, self.n).contiguous()
@@ -125,13 +202,35 @@ class EOBBatch:
     # ------------------------------------------------------------- trajectory
     def integrate(self, rtol=1e-12, atol=1e-12, hInit=0.01, step_back=10.0, fine_step=0.05, two_pass=True, fix_step=False,
                   mode="persistent", group=64, block=256, grid_groups=0, x0=None, step0=None, use_stop=True, tmax=1e13, done=None):
-        """Run the batch.  Returns the driver iteration count (plain mode) or 0.
+        """Run the batch.  Returns the driver iteration count (plain and coupled modes) or 0.
 
+        mode: "persistent" (one work-stealing launch), "plain" (one launch per
+        attempted step) -- both integrate each binary as a batch of one -- or
+        "coupled" (one launch per attempted step with the reference's batch-coupled
+        stopping criterion, ``eobhip_integrate_batch_coupled``; ``self.old_omegas``
+        then holds the criterion's omega history of the last pass).
         x0/step0 (device tensors [n]) continue an integration from row step0 of self.dense.
         done (pinned host int32 tensor [n], zeroed; persistent mode): done[b] becomes 1
         when binary b's outputs are in memory (``eobhip_integrate_signal``)."""
         if self.cond is None:
             raise RuntimeError("initial state not set")
+        if mode not in ("persistent", "plain", "coupled"):
+            raise ValueError(f"unknown mode {mode!r}")
+        if mode == "coupled":
+            if done is not None:
+                raise ValueError("completion flags need the persistent mode")
+            om = getattr(self, "old_omegas", None)
+            if om is None or tuple(om.shape) != (self.max_step, self.n):
+                om = torch.zeros((self.max_step, self.n), dtype=torch.float64, device=self.device)
+                self.old_omegas = om
+            it = ctypes.c_longlong(0)
+            call("eobhip_integrate_batch_coupled", self.cond.data_ptr(), self.bindata.data_ptr(), self.modes.data_ptr(), self.n,
+                 self.max_step, float(rtol), float(atol), float(hInit), float(step_back), float(fine_step), float(tmax),
+                 int(bool(two_pass)), int(bool(fix_step)), None if x0 is None else x0.data_ptr(),
+                 None if step0 is None else step0.data_ptr(), int(bool(use_stop)), int(group), self.dense.data_ptr(),
+                 self.num_steps.data_ptr(), self.stop_info.data_ptr(), self.pass1_steps.data_ptr(), self.attempts.data_ptr(),
+                 om.data_ptr(), self.scratch.data_ptr(), ctypes.byref(it), current_stream())
+            return int(it.value)
         if done is not None:
             if mode != "persistent":
                 raise ValueError("completion flags need the persistent mode")
@@ -650,14 +749,15 @@ class StreamedPass:
         if len(streams) < 2:
             raise RuntimeError(f"found {len(streams)} streams that run concurrently, the pass needs at least 2")
         self.integ_stream, self.down_streams = streams[0], streams[1:]
-        self.order = order
-        M = np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))[order].copy()
-        A = np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (n,))[order].copy()
-        self.batch = EOBBatch(make_additional_args(m1[order], m2[order], chi1[order], chi2[order]), max_step=max_step, device=device)
-        self.r0 = torch.as_tensor(r0[order].copy(), device=device)
-        self.M = torch.as_tensor(M, device=device)
-        self.amp = torch.as_tensor(A, device=device)
-        self.pos = torch.as_tensor(order, device=device)
+        # the binaries' parameters in pass order: host staging (pinned) and device rows
+        # m1, m2, chi1, chi2, r0, total mass, amplitude, caller position
+        self._stage = torch.empty((8, n), dtype=torch.float64, pin_memory=True)
+        self._par = torch.empty((8, n), dtype=torch.float64, device=device)
+        self._staged = None
+        self.r0, self.M, self.amp = self._par[4], self._par[5], self._par[6]
+        self.pos = torch.empty(n, dtype=torch.int64, device=device)
+        self.batch = EOBBatch.from_params(m1, m2, chi1, chi2, max_step=max_step, device=device)
+        self.set_params(m1, m2, chi1, chi2, r0, total_mass=total_mass, amp=amp, order=order)
         self.done = torch.zeros(n, dtype=torch.int32, pin_memory=True)
         self.rows = [_Rows(n, self.batch.max_step, device) for _ in self.down_streams]
         for st in self.down_streams:
@@ -671,6 +771,34 @@ class StreamedPass:
         for _ in range(int(warm_passes)):
             self.run()
             torch.cuda.synchronize(device)
+
+    def set_params(self, m1, m2, chi1, chi2, r0, total_mass=None, amp=None, order=None):
+        """New binaries (same count) for the next ``run``: masses, aligned spins,
+        initial separations [n]; total_mass (solar masses, for the sample spacing)
+        defaults to m1 + m2, amp to 1.  The pass order (longest expected first,
+        ``expected_length``) is computed here on the host; the parameters go to
+        the device in one copy on the current stream, and ``run`` builds the
+        additional args, tables and initial state from them on the device."""
+        n = self.n
+        m1, m2, chi1, chi2, r0 = (np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)) for x in (m1, m2, chi1, chi2, r0))
+        if order is None:
+            order = np.argsort(-expected_length(r0, m1, m2, chi1, chi2), kind="stable")   # longest first
+        order = np.asarray(order, dtype=np.int64)
+        if order.shape != (n,) or not np.array_equal(np.sort(order), np.arange(n)):
+            raise ValueError("order must be a permutation of range(n)")
+        M = m1 + m2 if total_mass is None else np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))
+        A = np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (n,))
+        if self._staged is not None:
+            self._staged.synchronize()   # the previous copy has left the staging rows
+        st = self._stage.numpy()
+        for j, x in enumerate((m1, m2, chi1, chi2, r0, M, A)):
+            st[j] = x[order]
+        st[7] = order
+        self._par.copy_(self._stage, non_blocking=True)
+        self.pos.copy_(self._par[7].to(torch.int64))
+        self._staged = torch.cuda.Event()
+        self._staged.record(torch.cuda.current_stream(self.device))
+        self.order = order
 
     def _launch(self, new, slot):
         rows, st, k = self.rows[slot], self.down_streams[slot], new.size
@@ -711,7 +839,8 @@ class StreamedPass:
         done[:] = 0   # the previous run's integrator raised every flag, so it writes none of these any more
         b = self.batch
         with torch.cuda.stream(self.integ_stream):
-            call("eobhip_bin_setup", b.add.data_ptr(), b.n, 1, b.bindata.data_ptr(), b.modes.data_ptr(), current_stream())
+            # additional args and tables from this pass's parameters (set_params)
+            b.set_params(self._par[0], self._par[1], self._par[2], self._par[3])
             b.initial_conditions(self.r0)
             b.integrate(rtol=self.rtol, atol=self.rtol, two_pass=True, group=group, block=block, grid_groups=grid_groups,
                         done=self.done)

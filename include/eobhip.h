@@ -129,6 +129,21 @@ size_t eobhip_bindata_bytes(void);
 size_t eobhip_modes_bytes(void);
 /* per-binary Hamiltonian + flux tables from additionalArgs [80][n] */
 int eobhip_bin_setup(const double* additionalArgs, int n, int use_hm, void* bindata, void* modes, void* stream);
+/* additionalArgs [80][n] built on the device from masses m1, m2 (any common unit;
+ * scaled to M = 1) and aligned spins chi1, chi2 [n], as SEOBNRv4PHM.setup_class
+ * builds it on the host (eob/seobnrv4phm.py:1922-1964: rows 0-3 the scaled masses
+ * and spins, row 4 omega0 = 0, rows 5-9 K, d5, dSO, dSS, h22_calib, rows 10-79 the
+ * Newtonian prefixes, Damour-Iyer-Nagar 2008 Eqs. 5-7), then the tables of
+ * eobhip_bin_setup, in one launch; bit for bit the host's
+ * eobgpu_amd.pipeline.make_additional_args.  Its reference counterpart is the
+ * host code of setup_class. */
+int eobhip_params_setup(const double* m1, const double* m2, const double* chi1, const double* chi2, int n, int use_hm, double K,
+                        double d5, double dSO, double dSS, double h22_calib, double* additionalArgs, void* bindata, void* modes,
+                        void* stream);
+/* The parameter-independent factors of the 35 Newtonian prefixes the device uses
+ * (host out, complex interleaved [35]); no reference counterpart (tests compare
+ * them with the host's). */
+int eobhip_newton_prefix_constants(double* out_host);
 
 /* Circular-orbit start at r0: pphi0 from dH/dr = 0, omega0 = dH/dpphi, and pr0 by the
  * reference's IC_diss false position on [-3e-2, 0] (src/EOB.cu:2666-2834). */
@@ -148,6 +163,8 @@ int eobhip_initial_state(const double* r0, const void* bindata, const void* mode
  * fixed-step pass of fine_step); two_pass = 0 one integrate() pass (fix_step as given).
  * x0/step0 (both NULL, or both [n]): continue from row step0 at time x0 (integrate()'s
  * step_num/denseOutput restart, eob/new_Dormand_horiz.py:504-520) with condBound holding that row;
+ * step0 in [-max_step, max_step - 2] (negative rows count from the end, as numpy's; rows
+ * past max_step - 1 are not stored);
  * use_stop = 0 runs without the StoppingCriterion (stop code 0);
  * dense [n][max_step][5] (t, r, phi, pr, pphi); num_steps[n] = stored points;
  * scratch: eobhip_integrate_scratch_bytes(n) bytes of device memory.
@@ -174,6 +191,24 @@ int eobhip_integrate_signal(const double* condBound, const void* bindata, const 
                             const double* x0, const int* step0, int use_stop, int group, int block, int grid_groups, double* dense,
                             int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts, void* scratch, int* done,
                             void* stream);
+
+/* eobhip_integrate with the reference's batch semantics of the stopping criterion:
+ * DOPR853.integrate's loop (eob/new_Dormand_horiz.py:585-775) with
+ * StoppingCriterion.__call__ (eob/seobnrv4phm.py:82-150) applied to the binaries
+ * accepted in each iteration in ascending order, whose third check reads the omega
+ * history at the binary's position in that list and which stores omegas only for
+ * the binaries past step 5 while any is; run_trajectory's two passes are
+ * batch-synchronous (eob/seobnrv4phm.py:2086-2123).  A binary's result then depends
+ * on its batch (eobhip_integrate treats each binary as a batch of one).  One launch
+ * per attempted step plus a one-block criterion launch and a 4-byte copy.
+ * old_omegas [max_step][n] (StoppingCriterion.old_omegas): zeroed at the start of each
+ * pass and left with the last pass's history.  Other arguments as eobhip_integrate
+ * (block 256, no grid_groups). */
+int eobhip_integrate_batch_coupled(const double* condBound, const void* bindata, const void* modes, int n, int max_step,
+                                   double rtol, double atol, double hInit, double step_back, double fine_step, double tmax,
+                                   int two_pass, int fix_step, const double* x0, const int* step0, int use_stop, int group,
+                                   double* dense, int* num_steps, int* stop_info, int* pass1_steps, unsigned long long* attempts,
+                                   double* old_omegas, void* scratch, long long* iterations_out, void* stream);
 
 /* (2,2) mode on the trajectory knots of every binary (get_hlms for (2,2)):
  * knots [n][max_step][3] = (phi, Re, Im) of h22 * exp(2 i phi); rows >= num_steps untouched */

@@ -212,6 +212,29 @@ def run_trajectory(condBound, add, coeffs, max_step=400, rtol=1e-12, atol=1e-12,
     return t.T.copy(), traj.transpose(2, 1, 0).copy(), ns, si
 
 
+def dopr_attempt(y, h, add, coeffs, rtol=1e-12, atol=1e-12):
+    """One attempted DOP853 step per lane: (k [12][4][n], y_new [4][n], err [n])."""
+    y = _c(y); h = _c(np.broadcast_to(np.asarray(h, dtype=np.float64), (y.shape[1],))); add = _c(add); coeffs = _c(coeffs)
+    n = y.shape[1]
+    k = np.zeros((12, 4, n)); yn = np.zeros((4, n)); err = np.zeros(n)
+    lib().orc_dopr_attempt(_p(y), _p(h), _p(add), n, _p(coeffs), _p(k), _p(yn), _p(err), ctypes.c_double(rtol),
+                           ctypes.c_double(atol))
+    return k, yn, err
+
+
+def hlm22(r, phi, v, Hreal, eta, vPhiIn, prefix, coeffs, h22_calib=0.0):
+    """EOBGetSpinFactorizedWaveform for (2,2) on knots [numSys][L] (src/EOB.cu:1616-1677):
+    prefix complex [numSys]; returns complex [numSys][L]."""
+    r = _c(r); phi = _c(phi); v = _c(v); Hreal = _c(Hreal); vPhiIn = _c(vPhiIn); eta = _c(eta); coeffs = _c(coeffs)
+    pf = _c(np.asarray(prefix, dtype=np.complex128).view(np.float64))
+    nb, L = r.shape
+    out = np.zeros((nb, L, 2))
+    idx = np.arange(nb, dtype=np.int32)
+    lib().orc_hlm22(_p(out), _p(r), _p(phi), _p(v), _p(Hreal), _p(eta), _p(vPhiIn), _p(pf), ctypes.c_double(h22_calib), nb, L,
+                    _p(coeffs), _pi(idx))
+    return out[..., 0] + 1j * out[..., 1]
+
+
 def error(solOld, h, ks, rtol=1e-12, atol=1e-12):
     """Restated error norm; ks = dict of stage arrays k1,k2,k3,k6..k10 ([4][n])."""
     solOld = _c(solOld); h = _c(h)
@@ -303,3 +326,92 @@ def likelihood_partials(hp, hc, data, psd, Fp, Fc, ts, df):
     out = np.zeros((nblk, nbin, nchan, 2))
     lib().orc_likelihood(_p(out), _p(hp), _p(hc), _p(data), _p(psd), _p(Fp), _p(Fc), _p(ts), ctypes.c_double(df), nbin, nchan, F)
     return out[..., 0] + 1j * out[..., 1]
+
+
+# ------------------------------------------------------------ whole pass
+MTSUN_SI = 4.925491025543575903411922162094833998e-6
+
+
+def psd_weight(f, f0=200.0):
+    """1/S(f) for S(f) = (f/f0)^-4 + 1 + (f/f0)^2, with 1/S(0) = 0."""
+    x = np.asarray(f, dtype=np.float64) / f0
+    with np.errstate(divide="ignore"):
+        S = x ** -4.0 + 1.0 + x * x
+    return np.where(np.asarray(f) > 0, 1.0 / S, 0.0)
+
+
+def hplus_from_trajectory(t, traj, ns, add, coeffs, total_mass, dt, N, amp=1.0):
+    """h+ = amp Re h22(t_j) on t_j = t_0 + j dt (seconds), j < N, for trajectories
+    t [n][L] (units of M), traj [n][4][L], ns[n] points: the (2,2) mode on the
+    knots (get_hlms, eob/seobnrv4phm.py:2138-2221), not-a-knot splines of phi and
+    Re, Im of h22 e^{2 i phi} (interpolate_TD, src/interpolate.cu:11-308),
+    evaluated on the uniform grid up to the last knot
+    (eob/seobnrv4phm.py:2319-2372).  Returns h [n][N]."""
+    n = len(ns)
+    amp = np.broadcast_to(np.asarray(amp, dtype=np.float64), (n,))
+    total_mass = np.broadcast_to(np.asarray(total_mass, dtype=np.float64), (n,))
+    h22 = get_hlm22(traj, ns, add, coeffs)
+    out = np.zeros((n, N))
+    for i in range(n):
+        L = int(ns[i])
+        tk = t[i, :L]
+        phi = traj[i, 1, :L]
+        s22 = h22[i, :L] * np.exp(2j * phi)
+        y = np.stack([phi, s22.real, s22.imag], axis=1)
+        B, c1, c2, c3 = interpolate_TD(tk[:, None].copy(), y, np.array([L], dtype=np.int32), 1, 3)
+        dT = dt / (total_mass[i] * MTSUN_SI)
+        nsamp = min(N, int((tk[-1] - tk[0]) / dT) + 1)
+        tj = tk[0] + np.arange(nsamp) * dT
+        ev = [spline_eval(tj, tk, y[:, q].copy(), c1[:, q].copy(), c2[:, q].copy(), c3[:, q].copy(), L) for q in range(3)]
+        out[i, :nsamp] = amp[i] * (ev[1] * np.cos(2 * ev[0]) + ev[2] * np.sin(2 * ev[0]))
+    return out
+
+
+def fd_logl(h, data, df, f0=200.0):
+    """log L = -1/2 4 df sum_k |d_k - rfft(h)_k|^2 / S(f_k) per row of h (the
+    reference's rfft without a dt factor and get_ll finish,
+    eob/seobnrv4phm.py:742-807)."""
+    ht = np.fft.rfft(h, axis=-1)
+    w = psd_weight(np.arange(ht.shape[-1]) * df, f0)
+    return -0.5 * 4 * df * np.sum(np.abs(data[None, :] - ht) ** 2 * w[None, :], axis=-1)
+
+
+def pipeline_logl(m1, m2, chi1, chi2, r0, data, dt, N, amp=1.0, max_step=4096, rtol=1e-10, atol=1e-10, hInit=0.01,
+                  fix_step1=False, f0=200.0):
+    """The whole pass for a batch: tables, circular initial conditions, two-pass
+    trajectory, (2,2) knots, splines, h+ on N samples, rfft and log L against
+    data (complex [N/2+1]).  Returns (logl, (t, traj, ns, stop_info), h)."""
+    m1 = np.atleast_1d(np.asarray(m1, dtype=np.float64))
+    add = make_additional_args(m1, m2, chi1, chi2)
+    co = fac_coeffs(add)
+    r0 = np.broadcast_to(np.asarray(r0, dtype=np.float64), m1.shape).copy()
+    pphi0, pr0, om0, add = initial_conditions(r0, add, co)
+    cond = np.array([r0, np.zeros(m1.size), pr0, pphi0])
+    t, traj, ns, si = run_trajectory(cond, add, co, max_step=max_step, rtol=rtol, atol=atol, hInit=hInit, fix_step1=fix_step1)
+    M = m1 + np.asarray(m2, dtype=np.float64)
+    h = hplus_from_trajectory(t, traj, ns, add, co, M, dt, N, amp)
+    df = 1.0 / (N * dt)
+    return fd_logl(h, data, df, f0), (t, traj, ns, si), h
+
+
+def fac_coeff_names():
+    """Field names of the oracle's FacCoeffs record in memory order (read from
+    the struct in oracle_flux.hpp), so tests can address coefficients by name."""
+    import re
+    text = open(os.path.join(HERE, "oracle_flux.hpp")).read()
+    body = text[text.index("struct FacCoeffs {"):]
+    body = body[:body.index("};")]
+    names = []
+    for line in body.splitlines():
+        line = line.split("//")[0].strip()
+        if line.startswith("double "):
+            names += [x.strip() for x in line[len("double "):].rstrip(";").split(",")]
+    assert len(names) == coeff_doubles(), (len(names), coeff_doubles())
+    return names
+
+
+def fac_coeffs_named(add, use_hm=1):
+    """fac_coeffs(add) as a list (one dict name -> value per binary)."""
+    co = fac_coeffs(add, use_hm)
+    names = fac_coeff_names()
+    return [dict(zip(names, row)) for row in co]

@@ -281,6 +281,22 @@ void orc_rhs(const double* arg, double* k, const double* add, int n, const doubl
     }
 }
 
+// One attempted DOP853 step per lane (src/DOPR853.cu:357-618): the twelve
+// stages kout [12][4][n], y_new [4][n] and the error norm err[n].
+void orc_dopr_attempt(const double* y, const double* h, const double* add, int n, const double* coeffs, double* kout, double* ynew,
+                      double* err, double rtol, double atol) {
+    const FacCoeffs* hc = reinterpret_cast<const FacCoeffs*>(coeffs);
+    for (int i = 0; i < n; ++i) {
+        Lane ln = {add, n, i, &hc[i]};
+        double yl[4] = {y[i], y[n + i], y[2 * n + i], y[3 * n + i]};
+        double k[12][4], yn[4];
+        dopr_attempt(ln, yl, h[i], k, false, yn, &err[i], rtol, atol);
+        for (int s = 0; s < 12; ++s)
+            for (int j = 0; j < 4; ++j) kout[((size_t)s * 4 + j) * n + i] = k[s][j];
+        for (int j = 0; j < 4; ++j) ynew[(size_t)j * n + i] = yn[j];
+    }
+}
+
 // error norm restated from src/DOPR853.cu:539-618 (same argument order as error_wrap)
 void orc_error(double* err, const double* solOld, double* solNew, const double* h, const double* k1, const double* k2,
                const double* k3, const double* k6, const double* k7, const double* k8, const double* k9, const double* k10,

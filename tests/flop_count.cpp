@@ -4,8 +4,9 @@
 // Conventions: + - * / count 1 each; sqrt, log, exp, cbrt, pow count 1 each and
 // are also reported separately as "transcendental".
 //
-//   hipcc -x c++ -D__HIP_PLATFORM_AMD__ -I/opt/rocm/include -O1 -std=c++17 -o flop_count flop_count.cpp
-// prints one JSON object; bench.py carries the frozen numbers it produced.
+//   g++ -x c++ -D__HIP_PLATFORM_AMD__ -I/opt/rocm/include -Ieobgpu_amd/csrc -O1 -std=c++17 -o flop_count tests/flop_count.cpp
+// prints one JSON object; bench.py carries the frozen step count, and
+// tests/test_derivations.py builds this file and checks that the two agree.
 #include <cmath>
 #include <cstdio>
 #include "eob_ham.hpp"

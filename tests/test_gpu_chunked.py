@@ -132,3 +132,51 @@ def test_gather_binaries(dev):
     np.testing.assert_array_equal(rows.amp[:k].cpu().numpy(), amp[idx].cpu().numpy())
     np.testing.assert_array_equal(rows.pos[:k].cpu().numpy(), pos[idx].cpu().numpy())
     np.testing.assert_array_equal(rows.dT_M[:k].cpu().numpy(), (dt / (M * P.MTSUN_SI))[idx].cpu().numpy())
+
+
+def test_params_setup_matches_host(dev):
+    """eobhip_params_setup builds the additional args and tables on the device bit
+    for bit as the host's make_additional_args + eobhip_bin_setup (equal masses
+    included, where the odd-m prefixes take their special values)."""
+    p = configs.uneven_batch(n=40, seed=12)
+    p["m2"][:3] = p["m1"][:3]
+    add = P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"])
+    b_host = P.EOBBatch(add, max_step=16, device=dev)
+    b_dev = P.EOBBatch.from_params(p["m1"], p["m2"], p["chi1"], p["chi2"], max_step=16, device=dev)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b_dev.add.cpu().numpy(), add)
+    np.testing.assert_array_equal(b_dev.bindata.cpu().numpy(), b_host.bindata.cpu().numpy())
+    np.testing.assert_array_equal(b_dev.modes.cpu().numpy(), b_host.modes.cpu().numpy())
+    # new parameters into the same buffers, from device tensors
+    q = configs.uneven_batch(n=40, seed=13)
+    b_dev.set_params(*(torch.tensor(q[k], device=dev) for k in ("m1", "m2", "chi1", "chi2")))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b_dev.add.cpu().numpy(), P.make_additional_args(q["m1"], q["m2"], q["chi1"], q["chi2"]))
+
+
+def test_streamed_pass_new_parameters(dev):
+    """StreamedPass.set_params: a pass over a new batch in the same buffers gives
+    that batch's one-batch numbers."""
+    n = 16
+    p = configs.uneven_batch(n=n, seed=14)
+    q = configs.uneven_batch(n=n, seed=15)
+    for d in (p, q):
+        d["r0"] = np.linspace(11.0, 15.0, n)[::-1].copy()
+    rng = np.random.default_rng(6)
+    F = N // 2 + 1
+    data = (rng.normal(size=F) + 1j * rng.normal(size=F)) * 1e-2
+    lik = P.SupportLikelihood(data, N, 1.0 / (N * DT), m_min_log2=6)
+    b = P.EOBBatch(P.make_additional_args(q["m1"], q["m2"], q["chi1"], q["chi2"]), max_step=2048, device=dev)
+    b.initial_conditions(q["r0"])
+    b.integrate(rtol=1e-10, atol=1e-10, two_pass=True, block=64)
+    b.hlm22_knots()
+    b.build_splines()
+    Mq = q["m1"] + q["m2"]
+    ref = lik(b, Mq, DT, amp=0.7).cpu().numpy()
+    sp = P.StreamedPass(p["m1"], p["m2"], p["chi1"], p["chi2"], p["r0"], lik, p["m1"] + p["m2"], DT, amp=0.7, max_step=2048)
+    sp.set_params(q["m1"], q["m2"], q["chi1"], q["chi2"], q["r0"], amp=0.7)
+    got = sp.run().cpu().numpy()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+    ns = sp.batch.num_steps.cpu().numpy()
+    np.testing.assert_array_equal(ns, b.num_steps.cpu().numpy()[sp.order])

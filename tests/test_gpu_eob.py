@@ -143,7 +143,19 @@ def test_initial_conditions(oracle, dev):
     pphi, pr, om = b.initial_conditions(p["r0"])
     np.testing.assert_allclose(pphi.cpu().numpy(), pphi_r, rtol=1e-13)
     np.testing.assert_allclose(om.cpu().numpy(), om_r, rtol=1e-12)
+    # pr0 is a false-position root with |residual| < 1e-12: equal to that tolerance
     np.testing.assert_allclose(pr.cpu().numpy(), pr_r, rtol=1e-8, atol=1e-13)
+    # the integrator's initial state written by the same launch: (r0, 0, pr0, pphi0)
+    c = b.cond.cpu().numpy()
+    np.testing.assert_array_equal(c[0], p["r0"])
+    assert not c[1].any()
+    np.testing.assert_array_equal(c[2], pr.cpu().numpy())
+    np.testing.assert_array_equal(c[3], pphi.cpu().numpy())
+    # the returned arrays are views of the batch's state: a second call rewrites them
+    om_first = om.clone()
+    b.initial_conditions(p["r0"] * 1.1)
+    assert (om.cpu().numpy() < om_first.cpu().numpy()).all()
+    np.testing.assert_array_equal(om.cpu().numpy(), b.add[4].cpu().numpy())
 
 
 def _run(add, cond, dev, max_step, **kw):
@@ -177,18 +189,24 @@ def test_trajectory_fixed_step_parity(oracle, dev, group, mode):
 # sits on its accept/reject boundary at almost every step, so the accepted-step
 # sequence is not reproducible between two floating-point implementations (the
 # oracle itself changes its step counts when one input moves by 1e-15).  The
-# adaptive trajectories are compared as solutions: at matched times, within
-# 10x the spread the oracle shows under that perturbation.
+# adaptive trajectories are compared as solutions: at matched times, within 3x
+# the spread the oracle shows when its initial state moves by one ulp
+# (tests/golden/adaptive_spread.json, written by make_adaptive_spread.py on
+# this same workload and comparison).
 def test_trajectory_adaptive_matched_times(oracle, dev):
+    import json
+    import os
     from scipy.interpolate import CubicSpline
+    spread = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "adaptive_spread.json")))
+    sp = spread["trajectory_16_binaries_rtol1e-10"]
     add, co, cond = _small_batch(oracle)
     max_step = 4096
     dL, dO, sn, si, _ = oracle.integrate(cond, add, co, max_step=max_step, rtol=1e-10, atol=1e-10)
     t, traj, ns, sig, _ = _run(add, cond, dev, max_step, rtol=1e-10, atol=1e-10, two_pass=False, group=64)
     steps_gpu = ns - 1
-    assert np.all(np.abs(steps_gpu - sn) <= 0.25 * sn), (steps_gpu, sn)
+    assert np.all(np.abs(steps_gpu - sn) <= 3 * sp["count"] * sn), (steps_gpu, sn)
     assert set(sig.tolist()) <= {3, 4, 5} and set(si.tolist()) <= {3, 4, 5}
-    tol = {0: 1e-4, 1: 5e-6, 3: 2e-5}
+    tol = {0: 3 * sp["r"], 1: 3 * sp["phi"], 3: 3 * sp["pphi"]}
     for b in range(cond.shape[1]):
         ta, ya = dL[:sn[b] + 1, b], dO[:sn[b] + 1, :, b]
         tb, yb = t[b, :ns[b]], traj[b, :, :ns[b]]

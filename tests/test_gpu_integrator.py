@@ -26,15 +26,21 @@ def _inputs(oracle, n=12, seed=1):
     return add, co, cond
 
 
-def _integrator():
+def _integrator(coupled):
     ode = SEOBNRv5Class()
-    return DOPR853(ode, stopping_criterion=StoppingCriterion(ode, MAX_STEP, use_gpu=True), tmax=1e13, max_step=MAX_STEP)
+    return DOPR853(ode, stopping_criterion=StoppingCriterion(ode, MAX_STEP, use_gpu=True), tmax=1e13, max_step=MAX_STEP,
+                   batch_coupled=coupled)
 
 
-def test_dopr853_dropin_single_pass(oracle, dev):
+# batch_coupled=True (the default) is the reference's batch semantics of the
+# stopping criterion; False integrates each binary as a batch of one.  The
+# oracle restates both.
+@pytest.mark.parametrize("coupled", [True, False])
+def test_dopr853_dropin_single_pass(oracle, dev, coupled):
     add, co, cond = _inputs(oracle)
-    loc_r, dense_r, sn_r, si_r, _ = oracle.integrate(cond, add, co, hInit=5.0, max_step=MAX_STEP, fix_step=True)
-    integ = _integrator()
+    loc_r, dense_r, sn_r, si_r, _ = oracle.integrate(cond, add, co, hInit=5.0, max_step=MAX_STEP, fix_step=True,
+                                                     batch_coupled=coupled)
+    integ = _integrator(coupled)
     loc, dense, sn = integ.integrate(cond.copy(), add, hInit=5.0, fix_step=True)
     assert loc.shape == (MAX_STEP, cond.shape[1]) and dense.shape == (MAX_STEP, 4, cond.shape[1])
     sn = sn.cpu().numpy()
@@ -49,13 +55,15 @@ def test_dopr853_dropin_single_pass(oracle, dev):
         assert not d[sn[b] + 1:, :, b].any()
 
 
-def test_dopr853_dropin_restart_matches_two_pass(oracle, dev):
+@pytest.mark.parametrize("coupled", [True, False])
+def test_dopr853_dropin_restart_matches_two_pass(oracle, dev, coupled):
     """The reference's run_trajectory glue (eob/seobnrv4phm.py:2091-2127) around two
     drop-in calls: a coarse pass, walk back 10 M from t[step_num - 1], then a
     fixed 0.05 M pass restarted from the row before the walk-back point."""
     add, co, cond = _inputs(oracle)
-    t_r, traj_r, ns_r, si_r = oracle.run_trajectory(cond, add, co, max_step=MAX_STEP, hInit=5.0, fix_step1=True)
-    integ = _integrator()
+    t_r, traj_r, ns_r, si_r = oracle.run_trajectory(cond, add, co, max_step=MAX_STEP, hInit=5.0, fix_step1=True,
+                                                    batch_coupled=coupled)
+    integ = _integrator(coupled)
     t, traj, step_num = integ.integrate(cond.copy(), add, hInit=5.0, fix_step=True)
     n = cond.shape[1]
     cols = torch.arange(n, device=t.device)

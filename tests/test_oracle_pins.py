@@ -275,6 +275,123 @@ def test_newtonian_prefix_22_closed_form(oracle):
     np.testing.assert_allclose(np.abs(pf[1]) * y22, 8 * math.sqrt(math.pi / 5) * nu, rtol=1e-14)
 
 
+def _published_coefficients(nu):
+    """Nonspinning factorized-resummation coefficients as published:
+    Damour-Iyer-Nagar 2008 (PRD 79, 064004) Eqs. 20-21 for delta_22 and the
+    rho_lm 1PN terms, Pan et al. 2011 (PRD 83, 064003) Eq. A8 and Fujita-Iyer
+    2010 (test-mass v^8, v^10 terms of rho_22) with the logarithms in
+    eulerlog_2(v) = gamma + ln(4 v)."""
+    pi = math.pi
+    return {
+        "delta22vh3": 7.0 / 3.0,
+        "delta22vh6": 428.0 * pi / 105.0,
+        "delta22vh9": -2203.0 / 81.0 + 1712.0 * pi ** 2 / 315.0,
+        "delta22v5": -24.0 * nu,
+        "rho22v2": -43.0 / 42.0 + 55.0 * nu / 84.0,
+        "rho22v3": 0.0,
+        "rho22v4": -20555.0 / 10584.0 - 33025.0 * nu / 21168.0 + 19583.0 * nu ** 2 / 42336.0,
+        "rho22v6": 1556919113.0 / 122245200.0 + (41.0 * pi ** 2 / 192.0 - 48993925.0 / 9779616.0) * nu
+                   - 6292061.0 / 3259872.0 * nu ** 2 + 10620745.0 / 39118464.0 * nu ** 3,
+        "rho22v6l": -428.0 / 105.0,
+        "rho22v8": -387216563023.0 / 160190110080.0,
+        "rho22v8l": 9202.0 / 2205.0,
+        "rho22v10": -16094530514677.0 / 533967033600.0,
+        "rho22v10l": 439877.0 / 55566.0,
+        "rho21v2": -59.0 / 56.0 + 23.0 * nu / 84.0,
+        "rho33v2": -7.0 / 6.0 + 2.0 * nu / 3.0,
+        "rho31v2": -13.0 / 18.0 - 2.0 * nu / 9.0,
+        "rho44v2": (1614.0 - 5870.0 * nu + 2625.0 * nu ** 2) / (1320.0 * (3.0 * nu - 1.0)),
+        "rho42v2": (1146.0 - 3530.0 * nu + 285.0 * nu ** 2) / (1320.0 * (3.0 * nu - 1.0)),
+        "rho66v2": (-106.0 + 602.0 * nu - 861.0 * nu ** 2 + 273.0 * nu ** 3) / (84.0 * (1.0 - 5.0 * nu + 5.0 * nu ** 2)),
+    }
+
+
+def test_factorized_waveform_coefficients_published_values(oracle):
+    """The (2,2) phase and amplitude tables and the 1PN rho_lm of the other modes
+    against the literature (nonspinning; odd-m modes for unequal masses only, the
+    reference leaves them unset at equal mass)."""
+    q = np.array([1.0, 1.7, 3.0, 8.0])
+    m1, m2 = q / (1 + q), 1 / (1 + q)
+    add = oracle.make_additional_args(m1, m2, np.zeros(q.size), np.zeros(q.size))
+    rows = oracle.fac_coeffs_named(add)
+    for i, c in enumerate(rows):
+        nu = m1[i] * m2[i]
+        for k, v in _published_coefficients(nu).items():
+            if q[i] == 1.0 and k in ("rho21v2", "rho33v2", "rho31v2"):
+                continue
+            assert abs(c[k] - v) <= 1e-14 * max(1.0, abs(v)), (q[i], k, c[k], v)
+
+
+def test_h22_on_knots_against_factorized_form(oracle):
+    """h_22 = h^N S_eff T_22 e^{i delta_22} rho_22^2 (Damour-Iyer-Nagar 2008 Eq. 2)
+    assembled here from independent pieces: T_22 = Gamma(3 - 2 i k) / 2 e^{pi k}
+    e^{2 i k ln(8 Omega / sqrt(e))} with k = 2 H Omega and scipy's complex Gamma,
+    delta_22 and rho_22 from the published series (nonspinning), S_eff = H_eff,
+    h^N = prefix_22 v_Phi^2 Y_2^{-2}(pi/2, phi) with scipy's Y_lm."""
+    for qq in (1.3, 4.0):
+        m1, m2 = qq / (1 + qq), 1 / (1 + qq)
+        add = oracle.make_additional_args(np.array([m1]), np.array([m2]), np.zeros(1), np.zeros(1))
+        co = oracle.fac_coeffs(add)
+        nu = m1 * m2
+        pub = _published_coefficients(nu)
+        pf = add[12, 0] + 1j * add[13, 0]
+        rng = np.random.default_rng(31)
+        L = 40
+        v = rng.uniform(0.15, 0.45, L)
+        Hr = 1.0 - nu * v ** 2 * rng.uniform(0.3, 0.6, L)
+        r = v ** -2 * rng.uniform(0.95, 1.05, L)
+        vPhiIn = rng.uniform(0.9, 1.1, L) / (v ** 6 * r ** 3)
+        phi = rng.uniform(0.0, 40.0, L)
+        h_o = oracle.hlm22(r[None], phi[None], v[None], Hr[None], np.array([nu]), vPhiIn[None], np.array([pf]), co)[0]
+        Om = v ** 3
+        k = 2.0 * Hr * Om
+        T = sps.gamma(3.0 - 2j * k) / 2.0 * np.exp(math.pi * k) * np.exp(2j * k * np.log(8.0 * Om / math.sqrt(math.e)))
+        y32 = Hr * Om
+        delta = 7.0 / 3.0 * y32 + pub["delta22vh6"] * y32 ** 2 + pub["delta22vh9"] * y32 ** 3 + pub["delta22v5"] * v ** 5
+        el = 0.5772156649015329 + np.log(4.0 * v)
+        rho = (1 + pub["rho22v2"] * v ** 2 + pub["rho22v4"] * v ** 4 + (pub["rho22v6"] + pub["rho22v6l"] * el) * v ** 6
+               + (pub["rho22v8"] + pub["rho22v8l"] * el) * v ** 8 + (pub["rho22v10"] + pub["rho22v10l"] * el) * v ** 10)
+        Seff = (Hr ** 2 - 1.0) / (2 * nu) + 1.0
+        vPhi = r * np.cbrt(vPhiIn) * Om
+        ylm = sps.sph_harm_y(2, -2, math.pi / 2, phi) if hasattr(sps, "sph_harm_y") else sps.sph_harm(-2, 2, phi, math.pi / 2)
+        hN = pf * vPhi ** 2 * ylm
+        h = hN * Seff * T * np.exp(1j * delta) * rho ** 2
+        np.testing.assert_allclose(h_o, h, rtol=1e-12)
+
+
+# ------------------------------------------------------ initial conditions
+def test_initial_condition_residuals(oracle):
+    """The circular start solves dH/dr = 0 at pr = 0 for pphi0, and the
+    dissipative condition of IC_diss (src/EOB.cu:2666-2688) for pr0:
+    dr/dt = dH/dpr with dr/dt = F_phi / (dpphi/dr) and dpphi/dr = -H_rr / H_rpphi
+    along circular orbits; omega0 = dH/dpphi.  The residuals are evaluated here
+    with the oracle's separately pinned gradient, Hessian and radiation-reaction
+    force."""
+    rng = np.random.default_rng(41)
+    n = 12
+    q = rng.uniform(1, 8, n)
+    add = oracle.make_additional_args(q / (1 + q), 1 / (1 + q), rng.uniform(-0.8, 0.8, n), rng.uniform(-0.8, 0.8, n))
+    co = oracle.fac_coeffs(add)
+    r0 = rng.uniform(12.0, 40.0, n)
+    pphi0, pr0, om0, add2 = oracle.initial_conditions(r0, add, co)
+    circ = np.array([r0, np.zeros(n), np.zeros(n), pphi0])
+    g0 = oracle.grad_H(circ, add)
+    assert np.all(np.abs(g0[0]) <= 1e-12 * np.abs(g0[3]) / r0), g0[0]
+    st = np.array([r0, np.zeros(n), pr0, pphi0])
+    g = oracle.grad_H(st, add)
+    hs = oracle.hessian_H(st, add)
+    aw = add.copy()
+    aw[4] = g[3]
+    F = oracle.rr_force(st, aw, co)
+    dLdr = -hs[0, 0] / hs[3, 0]
+    resid = F[1] / dLdr - g[2]
+    # the false position stops at |residual| < err = 1e-12 (absolute)
+    assert np.all(np.abs(resid) < 1e-12), resid
+    assert np.all(np.abs(resid) <= 1e-9 * np.abs(g[2])), resid / g[2]
+    np.testing.assert_allclose(om0, g0[3], rtol=1e-14)
+    assert np.all(pr0 < 0) and np.all(pr0 > -3e-2)
+
+
 # ------------------------------------------------------ special functions
 def test_complex_gamma_and_ylm(oracle):
     for z in (3 - 0.4j, 3 - 2.5j, 1.2 + 0.7j, 7.5 - 3j):
