@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-binaries", type=int, default=128)
+    ap.add_argument("--cpu-binaries", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.0)
@@ -228,19 +228,48 @@ def time_streamed(pipe, steps, warmup, group=64, block=64, grid_groups=0):
     return wall, logl_same, t_traj / steps, len(sp.groups), t_ing / steps * 1e3, attempts
 
 
-def cpu_baseline(pipe, nb):
-    """The whole pass on the CPU for the first nb binaries, one thread: the oracle's
-    initial conditions, two-pass trajectory and (2,2) knots, the oracle's
-    not-a-knot splines of the three knot series evaluated on the uniform grid
-    (up to each template's last knot), h+, numpy's rfft of N samples and the
-    likelihood sum against the same data.  Returns (likelihood evals/s, seconds)."""
+def cpu_info():
+    """(threads this process may use, capped at the 16 host cores one GPU's share
+    has on the GPU box; CPU model)."""
+    import os
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return min(n, 16), n, model
+
+
+def cpu_baseline(pipe, nb, threads=1):
+    """The whole pass on the CPU for the first nb binaries: the oracle's initial
+    conditions, two-pass trajectory and (2,2) knots, the oracle's not-a-knot
+    splines of the three knot series evaluated on the uniform grid (up to each
+    template's last knot), h+, numpy's rfft of N samples and the likelihood sum
+    against the same data.  threads > 1 splits the binaries over a thread pool
+    (the oracle's C code and numpy's FFT run without the interpreter lock).
+    Returns (likelihood evals/s, seconds, log L)."""
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     p = pipe.params
-    idx = np.arange(nb)
     data = pipe.data.cpu().numpy()
+    parts = [c for c in np.array_split(np.arange(nb), threads) if c.size]
+
+    def work(idx):
+        logl, _, _ = O.pipeline_logl(p["m1"][idx], p["m2"][idx], p["chi1"][idx], p["chi2"][idx], p["r0"][idx], data, pipe.dt,
+                                     pipe.N, amp=pipe.amp[idx], max_step=pipe.b.max_step, rtol=pipe.rtol, atol=pipe.rtol)
+        return logl
+
+    O.lib()
     t0 = time.perf_counter()
-    logl, _, _ = O.pipeline_logl(p["m1"][idx], p["m2"][idx], p["chi1"][idx], p["chi2"][idx], p["r0"][idx], data, pipe.dt, pipe.N,
-                                 amp=pipe.amp[idx], max_step=pipe.b.max_step, rtol=pipe.rtol, atol=pipe.rtol)
+    if threads == 1:
+        logl = work(np.arange(nb))
+    else:
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            logl = np.concatenate(list(ex.map(work, parts)))
     el = time.perf_counter() - t0
     return nb / el, el, logl
 
@@ -289,25 +318,19 @@ def workload_spline(args):
     n, N, dt = 512, 1 << args.log2N, 1.0 / 4096
     F, df = N // 2 + 1, 1.0 / (N * dt)
     Msun = 30.0
-    rng = np.random.default_rng(3)
-    K = rng.integers(5000, 60001, n)
+    from eobgpu_amd import configs
+    ts, phis, res, ims, K = configs.chirp_modes(n=n, seed=3, N=N, dt=dt, Msun=Msun)
+    rng = np.random.default_rng(33)
     kmax = int(K.max())
-    span = N * dt / (Msun * P.MTSUN_SI)                 # the data span in units of M
-    T = span * rng.uniform(0.05, 0.9, n)
     b = P.EOBBatch(P.make_additional_args(np.full(n, 0.6), np.full(n, 0.4), np.zeros(n), np.zeros(n)), max_step=kmax)
     dense = np.zeros((n, kmax, 5))
     knots = np.zeros((n, kmax, 3))
     for i in range(n):
-        u = np.sort(rng.uniform(0.0, 1.0, K[i]))
-        u[0], u[-1] = 0.0, 1.0 - 1e-6
-        t = T[i] * (1.0 - (1.0 - u) ** 0.375)
-        x = 1.0 - t / T[i]
-        f0 = 20.0 * Msun * P.MTSUN_SI                     # 20 Hz in units of 1/M
-        phi = np.pi * f0 * T[i] * 1.6 * (1.0 - x ** 0.625)  # orbital phase; GW frequency 20 Hz x^(-3/8)
-        dense[i, :K[i], 0] = t
-        knots[i, :K[i], 0] = phi
-        knots[i, :K[i], 1] = 1e-21 * x ** -0.25
-        knots[i, :K[i], 2] = 0.0
+        dense[i, :K[i], 0] = ts[i]
+        knots[i, :K[i], 0] = phis[i]
+        knots[i, :K[i], 1] = res[i]
+        knots[i, :K[i], 2] = ims[i]
+    del ts, phis, res, ims
     b.dense.copy_(torch.from_numpy(dense))
     b.knots = torch.from_numpy(knots).to(b.device)
     b.num_steps.copy_(torch.from_numpy(K.astype(np.int32)))
@@ -405,7 +428,44 @@ def workload_capacity(args):
            "attempted_steps_total": int(att), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
            "roofline": {"bound": "fp64-vector", "kernel": "k_integrate_persistent", "achieved": ach, "peak": PEAK_FP64_TFLOPS,
                         "unit": "TFLOP/s", "frac": ach / PEAK_FP64_TFLOPS}}
+    del b
+    torch.cuda.empty_cache()
+    out["served_by_footprint"] = capacity_served(p, args)
     print(json.dumps(out))
+
+
+def capacity_served(p, args, budget_gb=48.0, N=1 << 12, dt=1.0 / 2048):
+    """The same 16384 binaries as 64 requests of 256 arriving at once, served
+    with admission by projected footprint: size classes of 1024-8192 trajectory
+    rows under a memory budget, each binary in the smallest class its projected
+    length fits, refilled as slots free up (the full chain per binary,
+    likelihood at N = 2^12).  Reports throughput, the classes' memory against
+    one class sized for the longest binary, and the binaries run again after
+    filling their rows."""
+    import torch
+    from eobgpu_amd.serving import Request, Server, classes_for_budget, summarize, projected_rows, assign_classes, SlotPool
+    rng = np.random.default_rng(5)
+    F = N // 2 + 1
+    data = (rng.normal(size=F) + 1j * rng.normal(size=F)) * 1e-3
+    rows = (1024, 2048, 4096, 8192)
+    proj = projected_rows(p)
+    share = np.bincount(assign_classes(proj, rows), minlength=len(rows)).astype(np.float64)
+    classes = classes_for_budget(budget_gb * (1 << 30), N, class_rows=rows, weights=np.maximum(share, 1.0) * np.array(rows))
+    srv = Server(data, df=1.0 / (N * dt), dt=dt, N=N, classes=classes, streams=args.serve_streams)
+    srv.warm()
+    n = p["m1"].size
+    reqs = [Request(i, 0.0, {k: v[i * 256:(i + 1) * 256] for k, v in p.items()}) for i in range(n // 256)]
+    srv.run(reqs[:1])   # first use of the kernels and buffers
+    t0 = time.perf_counter()
+    res = srv.run(reqs)
+    wall = time.perf_counter() - t0
+    s = summarize(res)
+    uniform = n * SlotPool.bytes_per_binary_for(rows[-1], N)
+    return {"binaries": int(sum(r.nb for r in res)), "seconds": wall, "binaries_per_s": n / wall, "p50_ms": s["p50_ms"],
+            "p99_ms": s["p99_ms"], "classes": [{"max_step": ms, "capacity": c} for ms, c in classes],
+            "class_share_projected": share.astype(int).tolist(), "pool_bytes": int(srv.memory_bytes()),
+            "uniform_pool_bytes_for_all_at_8192_rows": int(uniform), "reruns": int(srv.reruns),
+            "logl_finite": bool(all(np.isfinite(r.logl).all() for r in res))}
 
 
 def main():
@@ -514,11 +574,19 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        rate, el, logl_cpu = cpu_baseline(pipe, args.cpu_binaries)
+        threads, nproc, model = cpu_info()
+        k1 = max(1, args.cpu_binaries // 4)
+        rate1, el1, logl1 = cpu_baseline(pipe, k1, threads=1)
         k = args.cpu_binaries
-        cpu = {"value": rate, "unit": "likelihood evals/s", "cores": 1, "kind": "port",
+        rate, el, logl_cpu = cpu_baseline(pipe, k, threads=threads)
+        print(f"[bench] cpu baseline: {rate:.1f} evals/s on {threads} threads, {rate1:.1f} on 1 ({model})", file=sys.stderr,
+              flush=True)
+        cpu = {"value": rate, "unit": "likelihood evals/s", "cores": threads, "kind": "port",
                "sample": f"whole pass (tables, initial conditions, trajectory, knots, splines, h+ on 2^{args.log2N} samples, "
-                         f"rfft, likelihood) for the first {k} binaries of the batch, oracle restatement + numpy, {el:.1f} s",
+                         f"rfft, likelihood) for the first {k} binaries of the batch on {threads} threads (oracle restatement + "
+                         f"numpy, binaries split over a thread pool), {el:.1f} s",
+               "cpu_model": model, "nproc": nproc,
+               "single_thread": {"value": rate1, "binaries": k1, "seconds": el1},
                "max_rel_diff_logl_vs_gpu": float(np.max(np.abs(logl_cpu - logl_support[:k]) / np.abs(logl_support[:k])))}
 
     out = {

@@ -33,6 +33,9 @@ _SIGS = {
     "eobhip_grad_Ham_align_AD": [c_vp, c_vp, c_vp, c_int, c_vp],
     "eobhip_hessian_Ham_align_AD": [c_vp, c_vp, c_vp, c_int, c_vp],
     "eobhip_IC_cons": [c_vp] * 5 + [c_int, c_vp],
+    "eobhip_root_find_all": [c_vp, c_vp, c_vp, c_vp, c_int, c_double, c_int, c_int, c_int, c_int, c_vp],
+    "eobhip_seobnr_IC_diss": [c_vp] * 9 + [c_int, c_vp],
+    "eobhip_seobnr_root_find_scalar_all": [c_vp] * 5 + [c_int, c_double, c_int, c_int, c_int, c_vp],
     "eobhip_seobnr_create": [c_vp],
     "eobhip_seobnr_destroy": [c_vp],
     "eobhip_seobnr_update_information": [c_vp] * 7 + [c_int, c_int, c_vp, c_vp],
@@ -77,6 +80,9 @@ _SIGS.update({
     "eobhip_spline_build_batched": [c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp],
     "eobhip_spline_scratch_doubles": [c_int, c_int],
     "eobhip_resample_hplus": [c_vp] * 5 + [c_int, c_int, c_int, c_vp, c_vp, c_vp],
+    "eobhip_resample_hpc": [c_vp] * 5 + [c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_detector_response": [c_vp] * 4 + [c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_detector_signal": [c_vp] * 5 + [c_int, c_int, c_int, c_double, c_int, c_vp, c_vp],
     "eobhip_rfft_batched": [c_vp, c_vp, c_int, c_int, c_vp],
     "eobhip_fd_likelihood": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_vp, c_vp],
     "eobhip_fd_likelihood_partial_doubles": [c_int, c_int],

@@ -183,6 +183,127 @@ __global__ __launch_bounds__(kBlock) void k_rr_force_dropin(double* force, doubl
     }
 }
 
+// ------------------------------------------- initial-condition solvers (drop-ins)
+// IC_cons at x = (r, pphi): (dH/dpphi - omega0, dH/dr) at (r, 0, 0, pphi), src/EOB.cu:405-430
+__device__ __forceinline__ void ic_cons_dev(const HamCoef& c, double omega0, double r, double L, double res[2]) {
+    res[0] = ham_tangent(c, r, 0.0, L, 2).d - omega0;
+    res[1] = ham_tangent(c, r, 0.0, L, 0).d;
+}
+
+// root_find_all (src/EOB.cu:2853-3031), one binary per thread: discrete Newton on
+// IC_cons with forward differences of step sqrt(2.22e-16) |x_j|, the 2x2
+// Jacobian inverted in closed form; it stops once every component's change
+// relative to its old value is below err, or after max_iter steps.
+__global__ __launch_bounds__(kBlock) void k_root_find_all(double* xOut, const double* x0In, const double* add, int max_iter, double err,
+                                                          int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const double om0 = add[4 * n + i];
+        const double eps = 2.22e-16;
+        double x0[2] = {x0In[i], x0In[n + i]}, x1[2] = {x0[0], x0[1]};
+        for (int it = 0; it < max_iter; ++it) {
+            x1[0] = x0[0];
+            x1[1] = x0[1];
+            double f[2], J[2][2];
+            ic_cons_dev(c, om0, x1[0], x1[1], f);
+            for (int j = 0; j < 2; ++j) {
+                const double dj = sqrt(eps) * fabs(x1[j]);
+                double xt[2] = {x1[0], x1[1]};
+                xt[j] += dj;
+                double f2[2];
+                ic_cons_dev(c, om0, xt[0], xt[1], f2);
+                J[0][j] = (f2[0] - f[0]) / dj;
+                J[1][j] = (f2[1] - f[1]) / dj;
+            }
+            const double det = 1. / (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+            const double inv[2][2] = {{det * J[1][1], det * -J[0][1]}, {det * -J[1][0], det * J[0][0]}};
+            for (int k = 0; k < 2; ++k) x1[k] -= inv[k][0] * f[0] + inv[k][1] * f[1];
+            double rel = 0.0;
+            for (int j = 0; j < 2; ++j) {
+                const double d = fabs(x1[j] - x0[j]) / x0[j];
+                rel = (d > rel) ? d : rel;
+            }
+            x0[0] = x1[0];
+            x0[1] = x1[1];
+            if (rel < err) break;
+        }
+        xOut[i] = x1[0];
+        xOut[n + i] = x1[1];
+    }
+}
+
+// IC_diss (src/EOB.cu:2666-2688) for binary i at pr: F_phi / (dpphi/dr) - dH/dpr
+// with dpphi/dr = -H_rr / H_rpphi and the force's omega = dH/dpphi at (r, pr, pphi).
+__device__ double ic_diss_dev(const ModeCoef* hm, const double* add, int n, int i, const HamCoef& c, double r, double pr, double L,
+                              double g[4], double F[2], double* hrr_out = nullptr, double* hrl_out = nullptr) {
+    g[0] = ham_tangent(c, r, pr, L, 0).d;
+    g[1] = 0.0;
+    g[2] = ham_tangent(c, r, pr, L, 1).d;
+    g[3] = ham_tangent(c, r, pr, L, 2).d;
+    const double hrr = ham_second(c, r, pr, L, 0, 0);
+    const double hrl = ham_second(c, r, pr, L, 0, 3);
+    if (hrr_out) *hrr_out = hrr;
+    if (hrl_out) *hrl_out = hrl;
+    rr_force_dropin(hm, add, n, i, c, r, pr, L, g[3], F);
+    const double dLdr = -hrr / hrl;
+    return F[1] / dLdr - g[2];
+}
+
+// IC_diss_kernel: args row 2 := pr, additional-args row 4 := dH/dpphi (both in
+// place, as the reference), grad_out [4][n], hess_out [16][n], force_out [2][n],
+// grad_temp_force [4][n] the gradient at pr = 0 that RR_force leaves.
+__global__ __launch_bounds__(kBlock) void k_ic_diss(double* out, const double* pr, double* args, double* add, double* grad,
+                                                    double* gtf, double* hess, double* force, int n, const ModeCoef* modes) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const double r = args[i], L = args[3 * n + i], p = pr[i];
+        args[2 * n + i] = p;
+        double g[4], F[2];
+        out[i] = ic_diss_dev(modes + (size_t)i * kNumModes, add, n, i, c, r, p, L, g, F);
+        for (int q = 0; q < 4; ++q) grad[q * n + i] = g[q];
+        const int v[3] = {0, 2, 3};
+        for (int q = 0; q < 16; ++q) hess[q * n + i] = 0.0;
+        for (int ia = 0; ia < 3; ++ia)
+            for (int ib = ia; ib < 3; ++ib) {
+                const double val = ham_second(c, r, p, L, v[ia], v[ib]);
+                hess[(v[ia] * 4 + v[ib]) * n + i] = val;
+                hess[(v[ib] * 4 + v[ia]) * n + i] = val;
+            }
+        add[4 * n + i] = g[3];
+        force[i] = F[0];
+        force[n + i] = F[1];
+        gtf[i] = ham_tangent(c, r, 0.0, L, 0).d;
+        gtf[n + i] = 0.0;
+        gtf[2 * n + i] = ham_tangent(c, r, 0.0, L, 1).d;
+        gtf[3 * n + i] = ham_tangent(c, r, 0.0, L, 3).d;
+    }
+}
+
+// root_find_scalar_all (src/EOB.cu:2712-2834), one binary per thread: false
+// position on IC_diss(pr) from start_bounds [2][n]; it stops when |IC_diss(c)| <
+// err or after max_iter steps.  The caller's args and additional args are read only.
+__global__ __launch_bounds__(kBlock) void k_root_find_scalar_all(double* pr_res, const double* bounds, const double* args,
+                                                                 const double* add, int max_iter, double err, int n,
+                                                                 const ModeCoef* modes) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const HamCoef c = ham_from_add(add, n, i);
+        const ModeCoef* hm = modes + (size_t)i * kNumModes;
+        const double r = args[i], L = args[3 * n + i];
+        double a = bounds[i], b = bounds[n + i], g[4], F[2];
+        double fa = ic_diss_dev(hm, add, n, i, c, r, a, L, g, F);
+        double fb = ic_diss_dev(hm, add, n, i, c, r, b, L, g, F);
+        double cc = 0.0;
+        for (int it = 0; it < max_iter; ++it) {
+            cc = (a * fb - b * fa) / (fb - fa);
+            const double fc = ic_diss_dev(hm, add, n, i, c, r, cc, L, g, F);
+            if ((fb < 0.0 && fc < 0.0) || (fb >= 0.0 && fc >= 0.0)) { b = cc; fb = fc; }
+            else { a = cc; fa = fc; }
+            if (fabs(fc) < err) break;
+        }
+        pr_res[i] = cc;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_update_info(ModeCoef* modes, WaveCoef* wave, const double* m1, const double* m2,
                                                         const double* eta, const double* a, const double* chiS, const double* chiA,
                                                         int use_hm, int n) {
@@ -861,6 +982,10 @@ __global__ __launch_bounds__(kBlock) void k_coupled_restart(IntegParams P, Plain
     store_state<G>(S, s, gl, P.nbin);
 }
 
+__global__ void k_zero_ints(int* a, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0;
+}
+
 __global__ void k_iota(int* a, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = i;
 }
@@ -1044,6 +1169,18 @@ int eobhip_IC_cons(double* res, const double* x, double* args, const double* add
     return check_launch("IC_cons");
 }
 
+int eobhip_root_find_all(double* xOut, const double* x0In, const double* argsIn, const double* add, int max_iter, double err,
+                         int numBinAll, int n, int num_args, int num_add_args, void* stream) {
+    (void)argsIn;
+    EOBHIP_REQUIRE(n == 2, "root_find_all: n must be 2 (r, pphi)");
+    EOBHIP_REQUIRE(num_args == 4 && num_add_args == kNAdd, "root_find_all: 4 args and 80 additional args per binary");
+    EOBHIP_REQUIRE(numBinAll >= 0 && max_iter >= 0, "bad sizes");
+    if (numBinAll == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_root_find_all, dim3(blocks_for(numBinAll, kBlock)), dim3(kBlock), 0, as_stream(stream), xOut, x0In, add,
+                       max_iter, err, numBinAll);
+    return check_launch("root_find_all");
+}
+
 int eobhip_seobnr_create(eobhip_seobnr_t* out) {
     EOBHIP_REQUIRE(out, "null handle pointer");
     *out = new eobhip_seobnr();
@@ -1095,6 +1232,28 @@ int eobhip_seobnr_RR_force(eobhip_seobnr_t h, double* force_out, double* grad_ou
     hipLaunchKernelGGL(k_rr_force_dropin, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), force_out, grad_out, args,
                        add, numSys, h->modes, idx);
     return check_launch("RR_force");
+}
+
+int eobhip_seobnr_IC_diss(eobhip_seobnr_t h, double* out, const double* pr, double* args, double* add, double* grad_out,
+                          double* grad_temp_force, double* hess_out, double* force_out, int numSys, void* stream) {
+    EOBHIP_REQUIRE(h && h->allocated, "IC_diss: update_information has not been called");
+    EOBHIP_REQUIRE(numSys >= 0 && numSys <= h->numSys, "IC_diss: numSys exceeds the handle's binaries");
+    if (numSys == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_ic_diss, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), out, pr, args, add, grad_out,
+                       grad_temp_force, hess_out, force_out, numSys, h->modes);
+    return check_launch("IC_diss");
+}
+
+int eobhip_seobnr_root_find_scalar_all(eobhip_seobnr_t h, double* pr_res, const double* start_bounds, const double* argsIn,
+                                       const double* add, int max_iter, double err, int numBinAll, int num_args, int num_add_args,
+                                       void* stream) {
+    EOBHIP_REQUIRE(h && h->allocated, "root_find_scalar_all: update_information has not been called");
+    EOBHIP_REQUIRE(num_args == 4 && num_add_args == kNAdd, "root_find_scalar_all: 4 args and 80 additional args per binary");
+    EOBHIP_REQUIRE(numBinAll >= 0 && numBinAll <= h->numSys && max_iter >= 0, "bad sizes");
+    if (numBinAll == 0) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_root_find_scalar_all, dim3(blocks_for(numBinAll, kBlock)), dim3(kBlock), 0, as_stream(stream), pr_res,
+                       start_bounds, argsIn, add, max_iter, err, numBinAll, h->modes);
+    return check_launch("root_find_scalar_all");
 }
 
 int eobhip_seobnr_ODE_Ham_align_AD(eobhip_seobnr_t h, const double* x, const double* arg, double* k, const double* add, int numSys,
@@ -1285,7 +1444,9 @@ int integrate_impl(const double* condBound, const void* bindata, const void* mod
     auto take = [&](size_t bytes) { void* p = base + off; off += (bytes + 255) & ~size_t(255); return p; };
     int* queue = static_cast<int*>(take(sizeof(int) * 2));
     P.queue = queue;
-    int rc = hip_status(hipMemsetAsync(queue, 0, sizeof(int) * 2, st), "memset queue");
+    // cleared by a kernel, not a memset node, so the persistent path can be captured in a HIP graph
+    hipLaunchKernelGGL(k_zero_ints, dim3(1), dim3(64), 0, st, queue, 2);
+    int rc = check_launch("k_zero_ints");
     if (rc) return rc;
 
     if (mode == 0) {

@@ -125,17 +125,12 @@ __host__ __device__ inline int spline_chunks(int len) {
     int P = len / kChunkMin;
     return P < 1 ? 1 : (P > kSplineThreads ? kSplineThreads : P);
 }
-// the largest chunk any len <= max_step gives: len itself below 2 kChunkMin,
-// under 1.5 kChunkMin up to 256 chunks, then ceil(len / 256)
-__host__ __device__ inline int spline_mmax(int max_step) {
-    const int small = max_step < 2 * kChunkMin ? max_step : 2 * kChunkMin;
-    const int big = (max_step + kSplineThreads - 1) / kSplineThreads;
-    return small > big ? small : big;
-}
 // per-interpolant scratch in doubles: slopes, the system's rows (4 per knot)
-// and per-chunk (c', d') rows
+// and the chunks' (c', d') rows, chunk p at 2 p (m + 2 kHalo) for m =
+// ceil(len / P): 2 P (m + 2 kHalo) <= 2 (len + P) + 256 P <= 4 len + 512,
+// since P = 1 or P <= len / kChunkMin, and P <= 256
 __host__ __device__ inline long long spline_scratch_per_interp(int max_step) {
-    return 5LL * max_step + (long long)kSplineThreads * 2 * (spline_mmax(max_step) + 2 * kHalo);
+    return 9LL * max_step + 512;
 }
 // rows whose loads a thread issues together in the serial sweeps: the sweeps
 // would otherwise wait one memory round trip per row
@@ -160,7 +155,6 @@ __global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const doubl
     }
     const int P = spline_chunks(len);
     const int m = (len + P - 1) / P;
-    const int mmax = spline_mmax(max_step);
     const int p = threadIdx.x;
     // every row of the system (sub-diagonal, diagonal, super-diagonal, right
     // side), by all threads at once
@@ -182,7 +176,7 @@ __global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const doubl
         if (s0 < e0) {
             const int lo = (s0 - kHalo > 0) ? s0 - kHalo : 0;
             const int hi = (e0 + kHalo < len) ? e0 + kHalo : len;
-            double* cw = S + 5 * (size_t)max_step + (size_t)p * 2 * (mmax + 2 * kHalo);   // c' and d' of rows lo..hi-1
+            double* cw = S + 5 * (size_t)max_step + (size_t)p * 2 * (m + 2 * kHalo);   // c' and d' of rows lo..hi-1
             double pc = 0.0, pd = 0.0;
             for (int i0 = lo; i0 < hi; i0 += kSweepBatch) {
                 double rl[kSweepBatch], rd[kSweepBatch], ru[kSweepBatch], rb[kSweepBatch];
@@ -250,12 +244,17 @@ __global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const doubl
 __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restrict__ dense, const double* __restrict__ coef,
                                                            const int* __restrict__ num_steps, const double* __restrict__ amp,
                                                            const double* __restrict__ dT_M, int n, int max_step, int N,
-                                                           double* __restrict__ hplus, int* __restrict__ nsamp) {
+                                                           double* __restrict__ hplus, int* __restrict__ nsamp,
+                                                           double* __restrict__ hcross) {
     const int b = blockIdx.y;
     const int len = num_steps[b];
     double* out = hplus + (size_t)b * N;
+    double* outc = hcross ? hcross + (size_t)b * N : nullptr;
     if (len < 2) {
-        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) out[j] = 0.0;
+        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) {
+            out[j] = 0.0;
+            if (outc) outc[j] = 0.0;
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = 0;
         return;
     }
@@ -268,7 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restr
     if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = (int)nb;
     const double a = amp[b];
     for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) {
-        double val = 0.0;
+        double val = 0.0, valc = 0.0;
         if (j < nb) {
             const double tj = t0 + (double)j * dT;
             int lo = 0, hi = len;
@@ -287,10 +286,12 @@ __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restr
             const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
             double sp, cp;
             sincos(2.0 * ph, &sp, &cp);
-            // Re[(sr + i si)(cos 2ph - i sin 2ph)]
+            // h+ - i hx = (sr + i si)(cos 2ph - i sin 2ph) (eob/seobnrv4phm.py:720, 742-744)
             val = a * (sr * cp + si * sp);
+            valc = a * (sr * sp - si * cp);
         }
         out[j] = val;
+        if (outc) outc[j] = valc;
     }
 }
 
@@ -494,8 +495,20 @@ int eobhip_resample_hplus(const double* dense, const double* coef, const int* nu
     int bx = (int)((N + (long long)kBlock * 4 - 1) / ((long long)kBlock * 4));
     if (bx > 4096) bx = 4096;
     hipLaunchKernelGGL(k_resample_hplus, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, n, max_step, N,
-                       hplus, nsamp);
+                       hplus, nsamp, nullptr);
     return check_launch("resample_hplus");
+}
+
+int eobhip_resample_hpc(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
+                        int max_step, int N, double* hplus, double* hcross, int* nsamp, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && N >= 1 && n <= 65535, "bad sizes");
+    EOBHIP_REQUIRE(hcross != nullptr, "hcross is required");
+    if (n == 0) return EOBHIP_OK;
+    int bx = (int)((N + (long long)kBlock * 4 - 1) / ((long long)kBlock * 4));
+    if (bx > 4096) bx = 4096;
+    hipLaunchKernelGGL(k_resample_hplus, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, n, max_step, N,
+                       hplus, nsamp, hcross);
+    return check_launch("resample_hpc");
 }
 
 // Batched real-to-complex FFT of n rows of N samples into n rows of N/2+1 bins (no dt factor,
@@ -526,3 +539,4 @@ size_t eobhip_fd_likelihood_partial_doubles(int n, int F) {
 }
 
 }  // extern "C"
+

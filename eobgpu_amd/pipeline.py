@@ -359,6 +359,14 @@ class SupportLikelihood:
              self.bank.data_ptr(), scratch.data_ptr(), current_stream())
         self._keep = (d, scratch)
         self._work = {}
+        self.grows = 0   # work-buffer reallocations so far (each may make the host wait for the device)
+
+    def reserve(self, n):
+        """Size the current stream's work buffers for calls of up to n templates of
+        any support (M = N each), so that no call on this stream allocates."""
+        gx, gx2 = ctypes.c_int(0), ctypes.c_int(0)
+        call("eobhip_lik_partial_blocks", self.log2N, ctypes.byref(gx), ctypes.byref(gx2))
+        self._buffers(n, n * self.N, n * (self.N // 2 + 1), gx.value, gx2.value)
 
     def _buffers(self, n, ph, px, gx, gx2):
         """Work buffers of the calls on the current stream, kept between calls and
@@ -370,6 +378,7 @@ class SupportLikelihood:
         if w is None or any(w["cap"][k] < v for k, v in need.items()):
             # grow by half again at least, so that calls of varying sizes reallocate rarely
             cap = {k: max(v, 0 if w is None else w["cap"][k] + w["cap"][k] // 2) for k, v in need.items()}
+            self.grows += 1
             dev = self.device
             w = {"cap": cap,
                  "nsamp": torch.empty(cap["n"], dtype=torch.int32, device=dev),
@@ -648,7 +657,7 @@ class _Rows:
         return v
 
 
-def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, poll_s=2e-5, spin_below=32):
+def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, poll_s=2e-5, spin_below=32, max_group=None):
     """The host's schedule in a streamed pass.
 
     done: host int array the integrator fills (nonzero = finished).
@@ -662,6 +671,7 @@ def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, po
     While more than ``spin_below`` binaries remain, polls sleep ``poll_s`` and
     ready second halves go before new groups; for the last ones, new groups
     go first.
+    A group takes at most ``max_group`` binaries (all newly finished ones if None).
     Raises if ``integrator_ended()`` while flags are missing.  Returns the
     groups (index arrays into done) in launch order."""
     n = done.size
@@ -679,6 +689,8 @@ def drain_finished(done, nslot, launch, complete, is_ready, integrator_ended, po
             continue
         if len(pending) < nslot:
             new = np.flatnonzero((done != 0) & ~taken)
+            if max_group is not None:
+                new = new[:max_group]
             if new.size:
                 taken[new] = True
                 left -= new.size
@@ -728,7 +740,7 @@ class StreamedPass:
     """
 
     def __init__(self, m1, m2, chi1, chi2, r0, lik, total_mass, dt, amp=None, max_step=4096, rtol=1e-10, device="cuda",
-                 order=None, poll_s=2e-5, down_streams=3, warm_passes=1):
+                 order=None, poll_s=2e-5, down_streams=3, warm_passes=1, max_group=128):
         m1, m2, chi1, chi2, r0 = (np.atleast_1d(np.asarray(x, dtype=np.float64)) for x in (m1, m2, chi1, chi2, r0))
         n = m1.size
         if n < 1:
@@ -759,10 +771,14 @@ class StreamedPass:
         self.batch = EOBBatch.from_params(m1, m2, chi1, chi2, max_step=max_step, device=device)
         self.set_params(m1, m2, chi1, chi2, r0, total_mass=total_mass, amp=amp, order=order)
         self.done = torch.zeros(n, dtype=torch.int32, pin_memory=True)
-        self.rows = [_Rows(n, self.batch.max_step, device) for _ in self.down_streams]
+        self.max_group = max(1, min(int(max_group), n))
+        self.rows = [_Rows(self.max_group, self.batch.max_step, device) for _ in self.down_streams]
         for st in self.down_streams:
             with torch.cuda.stream(st):
                 lik.warm_transforms()
+                # work buffers for the largest group, allocated now: an allocation in
+                # the middle of a pass can make the host wait for the integrator
+                lik.reserve(self.max_group)
         self.out = torch.empty(n, dtype=torch.float64, device=device)
         self.groups = []   # caller indices of each downstream group of the last run
         torch.cuda.synchronize(device)
@@ -849,11 +865,125 @@ class StreamedPass:
             if events is not None:
                 events["traj_end_longest"].record(self.integ_stream)
         groups = drain_finished(done, len(self.down_streams), self._launch, self._complete, lambda h: h[0]["ready"].query(),
-                                ended.query, self.poll_s)
+                                ended.query, self.poll_s, max_group=self.max_group)
         self.groups = [self.order[g] for g in groups]
         for st in (self.integ_stream, *self.down_streams):
             cur.wait_stream(st)
         return self.out
+
+
+class GraphPass:
+    """The whole pass for a fixed batch of n binaries captured once as a HIP graph
+    and replayed for every new batch of parameters.
+
+    The pass is the one-batch pass with the full-length likelihood: additional
+    args and tables from the parameters (``eobhip_params_setup``), circular
+    initial state, the two-pass trajectory in one persistent launch, the (2,2)
+    knots, the splines, h+ on N samples, the N-point transforms and the sum
+    against the data.  Every buffer is allocated when the object is built,
+    every input is read from device memory, and nothing in the pass waits for
+    the host, so the launches (about 20, the transforms included) are recorded
+    once and a pass is one graph launch; ``set_params`` copies the next batch's
+    parameters into the graph's input rows beforehand.  (The support-length
+    likelihood reads template sizes back to the host between its halves and
+    is not captured.)  A pass gives the numbers of the same kernels launched
+    one by one."""
+
+    def __init__(self, n, dtilde, N, dt, df=None, max_step=4096, rtol=1e-10, group=64, block=256, f0=200.0, device="cuda"):
+        L = lib()
+        self.n, self.N, self.dt, self.rtol, self.group, self.block, self.f0, self.device = (int(n), int(N), float(dt), float(rtol),
+                                                                                          int(group), int(block), float(f0), device)
+        self.F = self.N // 2 + 1
+        self.df = 1.0 / (self.N * self.dt) if df is None else float(df)
+        self.data = torch.as_tensor(dtilde, dtype=torch.complex128, device=device).contiguous()
+        if self.data.numel() != self.F:
+            raise ValueError("dtilde must have N/2 + 1 bins")
+        n = self.n
+        self.stream = torch.cuda.Stream(device=device)
+        # inputs: m1, m2, chi1, chi2, r0, dT_M, amp (device rows and their pinned staging)
+        self._stage = torch.empty((7, n), dtype=torch.float64, pin_memory=True)
+        self._staged = None
+        self.par = torch.zeros((7, n), dtype=torch.float64, device=device)
+        self.batch = EOBBatch.from_params(np.full(n, 0.6), np.full(n, 0.4), np.zeros(n), np.zeros(n), max_step=max_step,
+                                          device=device)
+        b = self.batch
+        b.knots = torch.zeros((n, b.max_step, 3), dtype=torch.float64, device=device)
+        b.coef = torch.zeros((n, b.max_step, 12), dtype=torch.float64, device=device)
+        b.spline_scratch = torch.zeros(int(L.eobhip_spline_scratch_doubles(n, b.max_step)), dtype=torch.float64, device=device)
+        self.cond = torch.zeros((4, n), dtype=torch.float64, device=device)
+        self.nsamp = torch.zeros(n, dtype=torch.int32, device=device)
+        self.hp = torch.zeros((n, self.N), dtype=torch.float64, device=device)
+        self.ht = torch.zeros((n, self.F), dtype=torch.complex128, device=device)
+        self.partial = torch.zeros(int(L.eobhip_fd_likelihood_partial_doubles(n, self.F)), dtype=torch.float64, device=device)
+        self.logl = torch.zeros(n, dtype=torch.float64, device=device)
+        self.graph = None
+
+    def set_params(self, m1, m2, chi1, chi2, r0, amp=1.0):
+        """The next batch: masses (solar masses), aligned spins, r0 [n], amplitude
+        scale; copied to the graph's input rows on the current stream."""
+        n = self.n
+        m1, m2, chi1, chi2, r0, amp = (np.broadcast_to(np.asarray(x, dtype=np.float64), (n,)) for x in (m1, m2, chi1, chi2, r0, amp))
+        if self._staged is not None:
+            self._staged.synchronize()
+        st = self._stage.numpy()
+        st[0], st[1], st[2], st[3], st[4] = m1, m2, chi1, chi2, r0
+        st[5] = self.dt / ((m1 + m2) * MTSUN_SI)
+        st[6] = amp
+        self.par.copy_(self._stage, non_blocking=True)
+        self._staged = torch.cuda.Event()
+        self._staged.record(torch.cuda.current_stream(self.device))
+
+    def _body(self):
+        b, n, s, par = self.batch, self.n, current_stream(), self.par
+        call("eobhip_params_setup", par[0].data_ptr(), par[1].data_ptr(), par[2].data_ptr(), par[3].data_ptr(), n, 1, 0.0, 0.0, 0.0,
+             0.0, 0.0, b.add.data_ptr(), b.bindata.data_ptr(), b.modes.data_ptr(), s)
+        call("eobhip_initial_state", par[4].data_ptr(), b.bindata.data_ptr(), b.modes.data_ptr(), n, self.cond.data_ptr(),
+             b.add[4].data_ptr(), 1000, 1e-12, s)
+        it = ctypes.c_longlong(0)
+        call("eobhip_integrate", self.cond.data_ptr(), b.bindata.data_ptr(), b.modes.data_ptr(), n, b.max_step, self.rtol, self.rtol,
+             0.01, 10.0, 0.05, 1e13, 1, 0, None, None, 1, 0, self.group, self.block, 0, b.dense.data_ptr(), b.num_steps.data_ptr(),
+             b.stop_info.data_ptr(), b.pass1_steps.data_ptr(), b.attempts.data_ptr(), b.scratch.data_ptr(), ctypes.byref(it), s)
+        call("eobhip_hlm22_knots", b.dense.data_ptr(), b.num_steps.data_ptr(), b.bindata.data_ptr(), b.modes.data_ptr(), n,
+             b.max_step, b.knots.data_ptr(), s)
+        call("eobhip_spline_build_batched", b.dense.data_ptr(), b.knots.data_ptr(), b.num_steps.data_ptr(), n, b.max_step,
+             b.coef.data_ptr(), b.spline_scratch.data_ptr(), s)
+        call("eobhip_resample_hplus", b.dense.data_ptr(), b.coef.data_ptr(), b.num_steps.data_ptr(), par[6].data_ptr(),
+             par[5].data_ptr(), n, b.max_step, self.N, self.hp.data_ptr(), self.nsamp.data_ptr(), s)
+        call("eobhip_rfft_batched", self.hp.data_ptr(), self.ht.data_ptr(), n, self.N, s)
+        call("eobhip_fd_likelihood", self.ht.data_ptr(), self.data.data_ptr(), self.partial.data_ptr(), n, self.F, self.df, self.f0,
+             self.logl.data_ptr(), s)
+
+    def run_eager(self):
+        """The same launches one by one on the pass's stream (the reference for the graph)."""
+        cur = torch.cuda.current_stream(self.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            self._body()
+        cur.wait_stream(self.stream)
+        return self.logl
+
+    def capture(self):
+        """Record the pass (after one eager pass on the same stream, which creates
+        the transform plans the pass uses)."""
+        self.run_eager()
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=self.stream):
+            self._body()
+        torch.cuda.synchronize(self.device)
+        self.graph = g
+        return g
+
+    def run(self):
+        """One pass as a graph launch; returns log L [n] (valid on the current stream)."""
+        if self.graph is None:
+            self.capture()
+        cur = torch.cuda.current_stream(self.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            self.graph.replay()
+        cur.wait_stream(self.stream)
+        return self.logl
 
 
 def analytic_psd(f, f0=200.0):

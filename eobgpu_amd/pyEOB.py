@@ -44,10 +44,13 @@ def IC_cons(res, x, args, additionalArgs, grad_out, numSys):
     sync()
 
 
-def root_find_all(*args, **kwargs):
-    """src/eob.pyx:50-70.  The 2-D discrete Newton initial-condition solve is not on the
-    batched path; use eobgpu_amd.pipeline.initial_conditions (circular start at r0)."""
-    raise NotImplementedError("root_find_all is not part of this build; see eobgpu_amd.pipeline.initial_conditions")
+@pointer_adjust
+def root_find_all(xOut, x0In, argsIn, additionalArgsIn, max_iter, err, numBinAll, n, num_args, num_add_args):
+    """src/eob.pyx:51-58 -> root_find_all_wrap (include/EOB.hh:6): (r, pphi) [2][numBinAll] from
+    the discrete Newton iteration on IC_cons, one binary per thread."""
+    call("eobhip_root_find_all", xOut, x0In, argsIn, additionalArgsIn, int(max_iter), float(err), int(numBinAll), int(n),
+         int(num_args), int(num_add_args), current_stream())
+    sync()
 
 
 class SEOBNRv5Class:
@@ -103,11 +106,19 @@ class SEOBNRv5Class:
              newtonian_prefix, float(h22_calib), int(numSys), int(traj_length), hCoeffs_index, current_stream())
         sync()
 
-    def IC_diss(self, *args, **kwargs):
-        raise NotImplementedError("IC_diss is not part of this build; see eobgpu_amd.pipeline.initial_conditions")
+    @pointer_adjust
+    def IC_diss(self, out, pr, args, additionalArgs, grad_out, grad_temp_force, hess_out, force_out, numSys):
+        """src/eob.pyx:133-144 -> IC_diss_wrap (include/EOB.hh:381)."""
+        call("eobhip_seobnr_IC_diss", self._h, out, pr, args, additionalArgs, grad_out, grad_temp_force, hess_out, force_out,
+             int(numSys), current_stream())
+        sync()
 
-    def root_find_scalar_all(self, *args, **kwargs):
-        raise NotImplementedError("root_find_scalar_all is not part of this build; see eobgpu_amd.pipeline.initial_conditions")
+    @pointer_adjust
+    def root_find_scalar_all(self, pr_res, start_bounds, argsIn, additionalArgsIn, max_iter, err, numBinAll, num_args, num_add_args):
+        """src/eob.pyx:169-176 -> root_find_scalar_all_wrap (include/EOB.hh:379)."""
+        call("eobhip_seobnr_root_find_scalar_all", self._h, pr_res, start_bounds, argsIn, additionalArgsIn, int(max_iter),
+             float(err), int(numBinAll), int(num_args), int(num_add_args), current_stream())
+        sync()
 
     def compute_hlms(self, *args, **kwargs):
         raise NotImplementedError("compute_hlms is dead code in the reference (src/EOB.cu:3319) and is not provided")

@@ -58,6 +58,13 @@ int eobhip_hessian_Ham_align_AD(const double* arg, double* hess_out, const doubl
 int eobhip_IC_cons(double* res, const double* x, double* args, const double* additionalArgs, double* grad_out, int numSys,
                    void* stream);
 
+/* root_find_all_wrap, include/EOB.hh:6; src/eob.pyx:51-58; src/EOB.cu:2853-3031.
+ * (r, pphi) solving IC_cons = 0 (dH/dpphi = omega0 from additionalArgs row 4, dH/dr = 0
+ * at pr = 0) by a discrete Newton iteration from x0In; xOut, x0In [2][numBinAll];
+ * n = 2, num_args = 4, num_add_args = 80 (argsIn is not read: IC_cons rebuilds the state). */
+int eobhip_root_find_all(double* xOut, const double* x0In, const double* argsIn, const double* additionalArgsIn, int max_iter,
+                         double err, int numBinAll, int n, int num_args, int num_add_args, void* stream);
+
 /* ------------------------------------------- eob.pyx SEOBNRv5Class (opaque handle) */
 typedef struct eobhip_seobnr* eobhip_seobnr_t;
 /* SEOBNRv5::SEOBNRv5 / ~SEOBNRv5, include/EOB.hh:372-374 */
@@ -75,6 +82,19 @@ int eobhip_seobnr_deallocate_information(eobhip_seobnr_t h);
 /* SEOBNRv5::RR_force_wrap, include/EOB.hh:378.  force_out [2][numSys]; omega taken from additionalArgs row 4 */
 int eobhip_seobnr_RR_force(eobhip_seobnr_t h, double* force_out, double* grad_out, const double* args,
                            const double* additionalArgs, int numSys, const int* hCoeffs_index, void* stream);
+/* SEOBNRv5::IC_diss_wrap, include/EOB.hh:381; src/EOB.cu:2666-2710, 3410-3421.
+ * out[numSys] = F_phi / (dpphi/dr) - dH/dpr at (args r, pr[i], args pphi); like the reference
+ * it writes pr into args row 2 and dH/dpphi into additionalArgs row 4, the gradient into
+ * grad_out [4][numSys], the Hessian into hess_out [16][numSys], the force into force_out
+ * [2][numSys] and the pr = 0 gradient into grad_temp_force [4][numSys]. */
+int eobhip_seobnr_IC_diss(eobhip_seobnr_t h, double* out, const double* pr, double* args, double* additionalArgs, double* grad_out,
+                          double* grad_temp_force, double* hess_out, double* force_out, int numSys, void* stream);
+/* SEOBNRv5::root_find_scalar_all_wrap, include/EOB.hh:379; src/EOB.cu:2712-2834, 3398-3408.
+ * pr_res[numBinAll] with IC_diss(pr) = 0 by false position from start_bounds [2][numBinAll];
+ * argsIn [4][numBinAll] (r, -, -, pphi) and additionalArgsIn [80][numBinAll] are read only. */
+int eobhip_seobnr_root_find_scalar_all(eobhip_seobnr_t h, double* pr_res, const double* start_bounds, const double* argsIn,
+                                       const double* additionalArgsIn, int max_iter, double err, int numBinAll, int num_args,
+                                       int num_add_args, void* stream);
 /* SEOBNRv5::ODE_Ham_align_AD_wrap, include/EOB.hh:380; src/EOB.cu:3486-3496.  k [4][numSys] */
 int eobhip_seobnr_ODE_Ham_align_AD(eobhip_seobnr_t h, const double* x, const double* arg, double* k,
                                    const double* additionalArgs, int numSys, const int* hCoeffs_index, void* stream);
@@ -227,6 +247,24 @@ int eobhip_spline_build_batched(const double* dense, const double* knots, const 
  * last knot; nsamp[n] = samples covered by the knots (eob/seobnrv4phm.py:2319-2372). */
 int eobhip_resample_hplus(const double* dense, const double* coef, const int* num_steps, const double* amp,
                           const double* dT_M, int n, int max_step, int N, double* hplus, int* nsamp, void* stream);
+/* eobhip_resample_hplus with the cross polarization as well: h+ - i hx = amp * h22(t_j)
+ * (the reference's template_channels, eob/seobnrv4phm.py:720, 742-744); hcross [n][N]. */
+int eobhip_resample_hpc(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
+                        int max_step, int N, double* hplus, double* hcross, int* nsamp, void* stream);
+
+/* ------------------------------------------------ detector projection
+ * BBHWaveformTD.get_detector_information (eob/seobnrv4phm.py:597-634): per binary b,
+ * theta = pi/2 - dec, phi = ra - gmst; Fplus, Fcross, time_shift [nifo][n] from the
+ * detector tensors [nifo][3][3] and vertices [nifo][3] (metres); gmst [n] is the
+ * caller's (the reference computes it with lal). */
+int eobhip_detector_response(const double* ra, const double* dec, const double* gmst, const double* psi, int n, const double* tensors,
+                             const double* vertices, int nifo, double* Fplus, double* Fcross, double* time_shift, void* stream);
+/* The "detector_fd" signal of BBHWaveformTD.__call__ (eob/seobnrv4phm.py:742-767):
+ * out [n][nifo][F] complex = (F+ hp~ + Fx hc~) exp(-2 pi i f dt), f = k df, zero for k < k_min;
+ * hp_fd, hc_fd complex [n][F]. */
+int eobhip_detector_signal(const double* hp_fd, const double* hc_fd, const double* Fplus, const double* Fcross, const double* time_shift,
+                           int n, int nifo, int F, double df, int k_min, double* out, void* stream);
+
 /* Row-wise real-to-complex FFT, n rows of N doubles -> n rows of N/2+1 complex (hipFFT D2Z;
  * like the reference's rfft without a dt factor, eob/seobnrv4phm.py:742-744). */
 int eobhip_rfft_batched(const double* in, double* out, int n, int N, void* stream);

@@ -171,6 +171,35 @@ def initial_conditions(r0, add, coeffs, max_iter=1000, err=1e-12):
     return pphi0, pr0, om0, add
 
 
+def root_find_all(x0, add, max_iter=1000, err=1e-12):
+    """root_find_all: (r, pphi) [2][n] solving IC_cons = 0 from x0 [2][n] (add row 4 = omega0)."""
+    x0 = _c(x0); add = _c(add)
+    n = x0.shape[1]
+    out = np.zeros((2, n))
+    lib().orc_root_find_all(_p(out), _p(x0), _p(add), int(max_iter), ctypes.c_double(err), n)
+    return out
+
+
+def ic_diss(pr, args, add, coeffs):
+    """IC_diss through its kernel: returns (out, args, add, grad, grad_temp_force, hess, force);
+    args and add are copies updated as the reference updates its arrays."""
+    pr = _c(np.atleast_1d(pr)); args = np.array(args, dtype=np.float64, order="C"); add = np.array(add, dtype=np.float64, order="C")
+    coeffs = _c(coeffs)
+    n = pr.size
+    out = np.zeros(n); g = np.zeros((4, n)); gt = np.zeros((4, n)); hs = np.zeros((16, n)); F = np.zeros((2, n))
+    lib().orc_ic_diss(_p(out), _p(pr), _p(args), _p(add), _p(g), _p(gt), _p(hs), _p(F), _p(coeffs), n)
+    return out, args, add, g, gt, hs, F
+
+
+def root_find_scalar_all(bounds, args, add, coeffs, max_iter=1000, err=1e-12):
+    """root_find_scalar_all: pr [n] with IC_diss(pr) = 0 in bounds [2][n]."""
+    bounds = _c(bounds); args = _c(args); add = _c(add); coeffs = _c(coeffs)
+    n = args.shape[1]
+    out = np.zeros(n)
+    lib().orc_root_find_scalar_all(_p(out), _p(bounds), _p(args), _p(add), _p(coeffs), int(max_iter), ctypes.c_double(err), n)
+    return out
+
+
 # --------------------------------------------------------------- integrator
 def integrate(condBound, add, coeffs, hInit=0.01, max_step=400, tmax=1e13, fix_step=False, rtol=1e-12, atol=1e-12,
               step_num=None, denseOutput=None, denseOutputLoc=None, batch_coupled=False):
@@ -232,6 +261,19 @@ def hlm22(r, phi, v, Hreal, eta, vPhiIn, prefix, coeffs, h22_calib=0.0):
     idx = np.arange(nb, dtype=np.int32)
     lib().orc_hlm22(_p(out), _p(r), _p(phi), _p(v), _p(Hreal), _p(eta), _p(vPhiIn), _p(pf), ctypes.c_double(h22_calib), nb, L,
                     _p(coeffs), _pi(idx))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def hlm(l, m, r, phi, pphi, v, Hreal, eta, vPhiIn, prefix, coeffs, h22_calib=0.0):
+    """EOBGetSpinFactorizedWaveform for (l, m) in {(2,2), (2,1), (3,3), (4,4), (5,5)} on knots
+    [numSys][L] (src/EOB.cu:1236-1613); prefix complex [numSys]; returns complex [numSys][L]."""
+    r = _c(r); phi = _c(phi); pphi = _c(pphi); v = _c(v); Hreal = _c(Hreal); vPhiIn = _c(vPhiIn); eta = _c(eta); coeffs = _c(coeffs)
+    pf = _c(np.asarray(prefix, dtype=np.complex128).view(np.float64))
+    nb, L = r.shape
+    out = np.zeros((nb, L, 2))
+    idx = np.arange(nb, dtype=np.int32)
+    lib().orc_hlm(_p(out), int(l), int(m), _p(r), _p(phi), _p(pphi), _p(v), _p(Hreal), _p(eta), _p(vPhiIn), _p(pf),
+                  ctypes.c_double(h22_calib), nb, L, _p(coeffs), _pi(idx))
     return out[..., 0] + 1j * out[..., 1]
 
 
@@ -415,3 +457,36 @@ def fac_coeffs_named(add, use_hm=1):
     co = fac_coeffs(add, use_hm)
     names = fac_coeff_names()
     return [dict(zip(names, row)) for row in co]
+
+
+# ---------------------------------------------------------- detector projection
+def detector_information(ra, dec, gmst, psi, tensors, vertices):
+    """(Fplus, Fcross, time_shift) [nifo][n]: get_detector_information
+    (eob/seobnrv4phm.py:597-634) restated with numpy, gmst given."""
+    phi = np.asarray(ra) - np.asarray(gmst)
+    theta = np.pi / 2 - np.asarray(dec)
+    psi = np.asarray(psi)
+    u = np.array([np.cos(phi) * np.cos(theta), np.cos(theta) * np.sin(phi), -np.sin(theta)]).T
+    v = np.array([-np.sin(phi), np.cos(phi), np.zeros_like(phi)]).T
+    m = -u * np.sin(psi)[:, None] - v * np.cos(psi)[:, None]
+    n = -u * np.cos(psi)[:, None] + v * np.sin(psi)[:, None]
+    ep = np.einsum("...i,...j->...ij", m, m) - np.einsum("...i,...j->...ij", n, n)
+    ec = np.einsum("...i,...j->...ij", m, n) + np.einsum("...i,...j->...ij", n, m)
+    D = np.asarray(tensors)
+    Fp = np.einsum("kij,lij->kl", D, ep)
+    Fc = np.einsum("kij,lij->kl", D, ec)
+    om = np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)]).T
+    dt = np.einsum("ij,kj->ki", om, np.asarray(vertices)) / 299792458.0
+    return Fp, Fc, dt
+
+
+def detector_signal(hp_fd, hc_fd, Fp, Fc, dt, df, k_min=0):
+    """[n][nifo][F] detector-frame signal (eob/seobnrv4phm.py:742-767)."""
+    F = hp_fd.shape[-1]
+    f = np.arange(F) * df
+    hp = np.array(hp_fd, dtype=np.complex128)
+    hc = np.array(hc_fd, dtype=np.complex128)
+    hp[:, :k_min] = 0.0
+    hc[:, :k_min] = 0.0
+    return (Fp.T[:, :, None] * hp[:, None, :] + Fc.T[:, :, None] * hc[:, None, :]) * np.exp(-1j * 2.0 * np.pi * f[None, None, :]
+                                                                                           * dt.T[:, :, None])

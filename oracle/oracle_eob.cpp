@@ -635,6 +635,74 @@ static cplx hlm22(double r, double phi, double v, double Hreal, const FacCoeffs&
     return h * hN;
 }
 
+// EOBGetSpinFactorizedWaveform (src/EOB.cu:1236-1613) for the modes the
+// higher-mode waveform uses, (2,2), (2,1), (3,3), (4,4), (5,5):
+// h_lm = T_lm e^{i delta_lm} S_lm (rho_lm^l + f_lm) h^N_lm, with rho^l replaced by
+// f_lm alone for odd m at eta = 1/4; S = H_eff for even l + m, v pphi for odd.
+static cplx hlm_mode(int l, int m, double r, double phi, double pphi, double v, double Hreal, const FacCoeffs& hc, cplx prefix,
+                     double eta, double vPhiInput, double h22_calib) {
+    if (l == 2 && m == 2) return hlm22(r, phi, v, Hreal, hc, prefix, eta, vPhiInput, h22_calib);
+    if (eta > 0.25 && eta < 0.25 + 1e-4) eta = 0.25;
+    const cplx I(0.0, 1.0);
+    const double v2 = v * v, Omega = v2 * v, vh3 = Hreal * Omega, vh = std::pow(vh3, 1. / 3.);
+    const double el = EG + std::log(2.0 * m * v);
+    double vPhi = r * std::pow(vPhiInput, 1.0 / 3.);
+    vPhi *= Omega;
+    const double vPhi2 = vPhi * vPhi;
+    const int eps = (l + m) % 2;
+    cplx hN = prefix * std::pow(vPhi2, (l + eps) / 2.0) * ylm_equator(l - eps, -m, phi);
+    const double Slm = (eps == 0) ? (Hreal * Hreal - 1.0) / (2.0 * eta) + 1.0 : v * pphi;
+    const double k = m * Omega, hk = Hreal * k;
+    const double z2 = std::tgamma((double)(l + 1));
+    cplx Tlm = std::exp(cplx(PI * hk, 2.0 * hk * std::log(4.0 * k / std::sqrt(std::exp(1.0))))) * cgamma(cplx(l + 1.0, -2.0 * hk));
+    Tlm /= z2;
+    double delta = 0.0, rho = 1.0;
+    cplx f = 0.0;
+    if (l == 2 && m == 1) {
+        delta = vh3 * (hc.delta21vh3 + vh3 * (hc.delta21vh6 + vh * (hc.delta21vh7 + hc.delta21vh9 * vh * vh))) + hc.delta21v5 * v * v2 * v2
+                + hc.delta21v7 * v2 * v2 * v2 * v;
+        rho = 1. + v * (hc.rho21v1 + v * (hc.rho21v2 + v * (hc.rho21v3 + v * (hc.rho21v4 + v * (hc.rho21v5 + v * (hc.rho21v6
+              + hc.rho21v6l * el + v * (hc.rho21v7 + hc.rho21v7l * el + v * (hc.rho21v8 + hc.rho21v8l * el
+              + (hc.rho21v10 + hc.rho21v10l * el) * v2))))))));
+        f = v * (hc.f21v1 + v2 * (hc.f21v3 + v * hc.f21v4 + v2 * (hc.f21v5 + v * hc.f21v6 + v2 * hc.f21v7c)));
+    } else if (l == 3 && m == 3) {
+        delta = vh3 * (hc.delta33vh3 + vh3 * (hc.delta33vh6 + hc.delta33vh9 * vh3)) + hc.delta33v5 * v * v2 * v2;
+        rho = 1. + v2 * (hc.rho33v2 + v * (hc.rho33v3 + v * (hc.rho33v4 + v * (hc.rho33v5 + v * (hc.rho33v6 + hc.rho33v6l * el
+              + v * (hc.rho33v7 + v * (hc.rho33v8 + hc.rho33v8l * el + v2 * (hc.rho33v10 + hc.rho33v10l * el))))))));
+        f = v * (v2 * (hc.f33v3 + v * (hc.f33v4 + v * (hc.f33v5 + v * hc.f33v6)))) + I * vh3 * vh3 * hc.f33vh6;
+    } else if (l == 4 && m == 4) {
+        delta = vh3 * (hc.delta44vh3 + vh3 * (hc.delta44vh6 + vh3 * hc.delta44vh9)) + hc.delta44v5 * v2 * v2 * v;
+        rho = 1. + v2 * (hc.rho44v2 + v * (hc.rho44v3 + v * (hc.rho44v4 + v * (hc.rho44v5 + v * (hc.rho44v6 + hc.rho44v6l * el
+              + v2 * (hc.rho44v8 + hc.rho44v8l * el + v2 * (hc.rho44v10 + hc.rho44v10l * el)))))));
+    } else if (l == 5 && m == 5) {
+        delta = vh3 * (hc.delta55vh3 + vh3 * (hc.delta55vh6 + vh3 * hc.delta55vh9)) + hc.delta55v5 * v2 * v2 * v;
+        rho = 1. + v2 * (hc.rho55v2 + v * (hc.rho55v3 + v * (hc.rho55v4 + v * (hc.rho55v5 + v * (hc.rho55v6 + hc.rho55v6l * el
+              + v2 * (hc.rho55v8 + hc.rho55v8l * el + v2 * (hc.rho55v10 + hc.rho55v10l * el)))))));
+        f = v2 * v * (hc.f55v3 + v * (hc.f55v4 + v * hc.f55v5c));
+    } else {
+        return cplx(NAN, NAN);
+    }
+    cplx rpl = (eta == 0.25 && (m % 2)) ? f : cplx(std::pow(rho, l), 0.0) + f;
+    cplx h = Tlm * std::exp(I * delta) * Slm * rpl;
+    return h * hN;
+}
+
+// EOBGetSpinFactorizedWaveform_wrap for (l, m) in {(2,2), (2,1), (3,3), (4,4), (5,5)}:
+// inputs per knot [numSys][traj_length]; prefix complex per binary.
+void orc_hlm(double* out, int l, int m, const double* r, const double* phi, const double* pphi, const double* v, const double* Hreal,
+             const double* eta, const double* vPhiIn, const double* prefix, double h22_calib, int numSys, int traj_length,
+             const double* coeffs, const int* idx) {
+    const FacCoeffs* hc = reinterpret_cast<const FacCoeffs*>(coeffs);
+    for (int b = 0; b < numSys; ++b) {
+        cplx pf(prefix[2 * b], prefix[2 * b + 1]);
+        for (int s = 0; s < traj_length; ++s) {
+            size_t q = (size_t)b * traj_length + s;
+            cplx h = hlm_mode(l, m, r[q], phi[q], pphi[q], v[q], Hreal[q], hc[idx[b]], pf, eta[b], vPhiIn[q], h22_calib);
+            out[2 * q] = h.real(); out[2 * q + 1] = h.imag();
+        }
+    }
+}
+
 // EOBGetSpinFactorizedWaveform_wrap restricted to (l,m) = (2,2),
 // src/EOB.cu:1616-1677: inputs per knot [numSys][traj_length].
 void orc_hlm22(double* out /* complex interleaved */, const double* r, const double* phi, const double* v, const double* Hreal,
@@ -838,6 +906,117 @@ void orc_initial_conditions(const double* r0, double* add, const double* coeffs,
         pphi0[i] = L;
         pr0[i] = c;
         add[4 * n + i] = omega0[i];
+    }
+}
+
+// IC_cons (src/EOB.cu:405-430): res = (dH/dpphi - omega0, dH/dr) at (r, 0, 0, pphi),
+// omega0 from additional-args row 4; one lane of a batch of n.
+static void ic_cons1(const double x[2], const double* add, int n, int i, double res[2]) {
+    HamParams p = params_of(add, n, i);
+    double g[4];
+    grad1(x[0], 0.0, x[1], p, g);
+    res[0] = g[3] - add[4 * n + i];
+    res[1] = g[0];
+}
+
+// root_find_all (src/EOB.cu:2853-3031): per binary, a discrete Newton iteration
+// on IC_cons in x = (r, pphi): forward differences of step sqrt(2.22e-16) |x_j|,
+// the 2x2 Jacobian inverted in closed form, x -= J^-1 f; it stops once every
+// component's change relative to its old value is below err, or after max_iter.
+// x0In, xOut: [2][n].
+void orc_root_find_all(double* xOut, const double* x0In, const double* add, int max_iter, double err, int n) {
+    const double eps = 2.22e-16;
+    for (int i = 0; i < n; ++i) {
+        double x0[2] = {x0In[i], x0In[n + i]}, x1[2] = {x0[0], x0[1]};
+        for (int it = 0; it < max_iter; ++it) {
+            x1[0] = x0[0]; x1[1] = x0[1];
+            double f[2], J[2][2];
+            ic_cons1(x1, add, n, i, f);
+            for (int k = 0; k < 2; ++k)
+                for (int j = 0; j < 2; ++j) {
+                    const double dj = std::sqrt(eps) * std::fabs(x1[j]);
+                    double xt[2] = {x1[0], x1[1]};
+                    xt[j] += dj;
+                    double f2[2];
+                    ic_cons1(xt, add, n, i, f2);
+                    J[k][j] = (f2[k] - f[k]) / dj;
+                }
+            const double det = 1. / (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+            const double inv[2][2] = {{det * J[1][1], det * -J[0][1]}, {det * -J[1][0], det * J[0][0]}};
+            for (int k = 0; k < 2; ++k) {
+                double t = 0.0;
+                for (int j = 0; j < 2; ++j) t += inv[k][j] * f[j];
+                x1[k] -= t;
+            }
+            double rel = 0.0;
+            for (int j = 0; j < 2; ++j) {
+                const double d = std::fabs(x1[j] - x0[j]) / x0[j];
+                if (d > rel) rel = d;
+            }
+            x0[0] = x1[0]; x0[1] = x1[1];
+            if (rel < err) break;
+        }
+        xOut[i] = x1[0];
+        xOut[n + i] = x1[1];
+    }
+}
+
+// IC_diss (src/EOB.cu:2666-2688) through its kernel (IC_diss_kernel): per
+// binary, args row 2 := pr, the gradient and Hessian of H there, additional-args
+// row 4 := dH/dpphi, the radiation-reaction force with that omega (force_out
+// [2][n], grad_temp_force [4][n] = the gradient at pr = 0 RR_force leaves), and
+// out = F_phi / (dpphi/dr) - dH/dpr with dpphi/dr = -H_rr / H_rpphi.  args and
+// add are updated in place as the reference's kernel does.
+void orc_ic_diss(double* out, const double* pr, double* args, double* add, double* grad_out, double* grad_temp_force,
+                 double* hess_out, double* force_out, const double* coeffs, int n) {
+    const FacCoeffs* hc = reinterpret_cast<const FacCoeffs*>(coeffs);
+    for (int i = 0; i < n; ++i) {
+        args[2 * n + i] = pr[i];
+        HamParams p = params_of(add, n, i);
+        double g[4], a1[4] = {args[i], args[n + i], args[2 * n + i], args[3 * n + i]}, hs[16];
+        grad1(a1[0], a1[2], a1[3], p, g);
+        double add1[NADD];
+        for (int j = 0; j < NADD; ++j) add1[j] = add[j * n + i];
+        orc_hessian_H(a1, hs, add1, 1);
+        for (int j = 0; j < 4; ++j) grad_out[j * n + i] = g[j];
+        for (int j = 0; j < 16; ++j) hess_out[j * n + i] = hs[j];
+        add[4 * n + i] = g[3];
+        double F[2];
+        rr_force1(a1, add, n, i, g[3], hc[i], F);
+        double gc[4];
+        grad1(a1[0], 0.0, a1[3], p, gc);
+        for (int j = 0; j < 4; ++j) grad_temp_force[j * n + i] = gc[j];
+        force_out[i] = F[0];
+        force_out[n + i] = F[1];
+        const double dLdr = -hs[0] / hs[12];
+        out[i] = F[1] / dLdr - g[2];
+    }
+}
+
+// root_find_scalar_all (src/EOB.cu:2712-2834): per binary, false position on
+// IC_diss(pr) in [start_bounds[0], start_bounds[1]] (rows [2][n]) with the
+// binary's args and additional args copied (IC_diss's writes stay in the copy);
+// it stops when |IC_diss(c)| < err or after max_iter steps.
+void orc_root_find_scalar_all(double* pr_res, const double* bounds, const double* args, const double* add, const double* coeffs,
+                              int max_iter, double err, int n) {
+    const FacCoeffs* hc = reinterpret_cast<const FacCoeffs*>(coeffs);
+    for (int i = 0; i < n; ++i) {
+        double add1[NADD];
+        for (int j = 0; j < NADD; ++j) add1[j] = add[j * n + i];
+        const double r = args[i], L = args[3 * n + i];
+        double a = bounds[i], b = bounds[n + i];
+        // ic_diss1 evaluates IC_diss with omega = dH/dpphi at that pr, as the reference's copy is updated
+        double fa = ic_diss1(a, r, L, add1, 1, 0, hc[i]);
+        double fb = ic_diss1(b, r, L, add1, 1, 0, hc[i]);
+        double c = 0.0;
+        for (int it = 0; it < max_iter; ++it) {
+            c = (a * fb - b * fa) / (fb - fa);
+            const double fc = ic_diss1(c, r, L, add1, 1, 0, hc[i]);
+            if ((fb < 0.0 && fc < 0.0) || (fb >= 0.0 && fc >= 0.0)) { b = c; fb = fc; }
+            else { a = c; fa = fc; }
+            if (std::fabs(fc) < err) break;
+        }
+        pr_res[i] = c;
     }
 }
 

@@ -51,6 +51,11 @@ struct FacCoeffs {
     double rho77v2, rho77v3, rho76v2, rho75v2, rho75v3, rho74v2, rho73v2, rho73v3, rho72v2, rho71v2, rho71v3;
     // (8,m)
     double rho88v2, rho87v2, rho86v2, rho85v2, rho84v2, rho83v2, rho82v2, rho81v2;
+    // phases of the higher modes the waveform evaluates (2,1), (3,3), (4,4), (5,5)
+    double delta21vh3, delta21vh6, delta21vh7, delta21vh9, delta21v5, delta21v7;
+    double delta33vh3, delta33vh6, delta33vh9, delta33v5;
+    double delta44vh3, delta44vh6, delta44vh9, delta44v5;
+    double delta55vh3, delta55vh6, delta55vh9, delta55v5;
 };
 
 // src/EOB.cu:1946-2462
@@ -339,6 +344,26 @@ inline void calc_fac_coeffs(FacCoeffs* c, int use_hm, double m1, double m2, doub
     if (unequal) c->rho83v2 = (20598.0 - 131059.0 * eta + 249018.0 * eta2 - 149950.0 * eta3 + 24520.0 * eta2 * eta2) / (18240.0 * (-1.0 + 6.0 * eta - 10.0 * eta2 + 4.0 * eta3));
     c->rho82v2 = (2462.0 - 17598.0 * eta + 37119.0 * eta2 - 22845.0 * eta3 + 3063.0 * eta2 * eta2) / (2736.0 * (-1.0 + 7.0 * eta - 14.0 * eta2 + 7.0 * eta3));
     if (unequal) c->rho81v2 = (20022.0 - 126451.0 * eta + 236922.0 * eta2 - 138430.0 * eta3 + 21640.0 * eta2 * eta2) / (18240.0 * (-1.0 + 6.0 * eta - 10.0 * eta2 + 4.0 * eta3));
+
+    // phases delta_lm of (2,1), (3,3), (4,4), (5,5) (src/EOB.cu:2019-2023, 2080-2083,
+    // 2178-2185, 2263-2271); aDelta = 0 as in the reference.  delta21v7, delta44v5
+    // and delta55v5 are read by the waveform but never set by the reference (its
+    // record is uninitialised memory); they are 0 here.
+    c->delta21vh3 = 2.0 / 3.0;
+    c->delta21vh6 = (107.0 * PI) / 105.0;
+    c->delta21vh7 = 0.0;
+    c->delta21vh9 = -272.0 / 81.0 + (214.0 * PI * PI) / 315.0;
+    c->delta21v5 = -493.0 * eta / 42.0;
+    c->delta33vh3 = 13.0 / 10.0;
+    c->delta33vh6 = (39.0 * PI) / 7.0;
+    c->delta33vh9 = -227827.0 / 3000.0 + (78.0 * PI * PI) / 7.0;
+    c->delta33v5 = -80897.0 * eta / 2430.0;
+    c->delta44vh3 = (112.0 + 219.0 * eta) / (-120.0 * m1p3);
+    c->delta44vh6 = (25136.0 * PI) / 3465.0;
+    c->delta44vh9 = use_hm ? -55144.0 / 375.0 + 201088.0 * PI * PI / 10395.0 : 0.0;
+    c->delta55vh3 = (96875.0 + 857528.0 * eta) / (131250.0 * (1 - 2 * eta));
+    c->delta55vh6 = use_hm ? 3865.0 / 429.0 * PI : 0.0;
+    c->delta55vh9 = use_hm ? (-7686949127.0 + 954500400.0 * PI * PI) / 31783752.0 : 0.0;
 }
 
 // |Y_{l}^{-m}(theta=pi/2, phi=0)| for 0<=m<=l<=8 from the closed form

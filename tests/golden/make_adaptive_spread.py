@@ -15,6 +15,10 @@ root:  python tests/golden/make_adaptive_spread.py
     count, and of r, phi, pphi at matched times (cubic splines of both
     trajectories on 1500 times up to 98 % of the shorter one, relative to the
     component's largest magnitude);
+  uneven trajectories: the first 64 binaries of the 1024-binary workload
+    (q ~ U[1,10], chi ~ U[-0.9,0.9], log10 M ~ U[1,2], r0 ~ U[15,40] M, seed 2),
+    two-pass adaptive runs; recorded: which binaries keep their stop code, and
+    the largest relative change of the stored points of those binaries;
   log L: the first 32 binaries of the 1024-binary workload (seed 2), the whole
     pass (two-pass trajectory at rtol = 1e-10, (2,2) knots, splines, h+ on 2^20
     samples at 1/4096 s, rfft, log L against binary 0 injected at SNR 20 plus
@@ -71,6 +75,28 @@ def trajectory_spread():
     return worst
 
 
+def uneven_trajectory_spread(nb=64):
+    """Two-pass adaptive trajectories of the first nb binaries of the 1024-binary
+    workload: the largest relative change of a binary's stored points under the
+    one-ulp perturbations, and which binaries keep their stop code under all of
+    them."""
+    p = configs.uneven_batch(n=1024, seed=2)
+    idx = np.arange(nb)
+    add = O.make_additional_args(p["m1"][idx], p["m2"][idx], p["chi1"][idx], p["chi2"][idx])
+    co = O.fac_coeffs(add)
+    pphi0, pr0, om0, add = O.initial_conditions(p["r0"][idx], add, co)
+    cond = np.array([p["r0"][idx], np.zeros(nb), pr0, pphi0])
+    base = O.run_trajectory(cond, add, co, max_step=4096, rtol=1e-10, atol=1e-10)
+    runs = [O.run_trajectory(c, add, co, max_step=4096, rtol=1e-10, atol=1e-10) for c in perturbations(cond)]
+    stable = np.ones(nb, dtype=bool)
+    for t, traj, ns, si in runs:
+        stable &= si == base[3]
+    # a changed stop code moves the end of pass 1 and with it the whole count; the
+    # count spread is taken over the binaries whose code does not change
+    worst = max(float(np.max(np.abs(ns - base[2])[stable] / base[2][stable])) for t, traj, ns, si in runs)
+    return {"count": worst, "stable_codes": stable.astype(int).tolist()}
+
+
 def logl_spread(nb=32):
     p = configs.uneven_batch(n=1024, seed=2)
     idx = np.arange(nb)
@@ -109,6 +135,8 @@ def logl_spread(nb=32):
 
 if __name__ == "__main__":
     out = {"trajectory_16_binaries_rtol1e-10": trajectory_spread()}
+    print(out, flush=True)
+    out["trajectory_64_uneven_rtol1e-10"] = uneven_trajectory_spread()
     print(out, flush=True)
     out["logl_rel_32_binaries_uneven_N2^20"] = logl_spread()
     print(out, flush=True)

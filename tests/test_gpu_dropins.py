@@ -255,3 +255,55 @@ def test_reference_driver_through_dropins(oracle, dev):
     assert np.array_equal(b.num_steps.cpu().numpy() - 1, sn_r) and np.array_equal(b.stop_info.cpu().numpy(), si_r)
     assert itk == it_r
     ode.deallocate_information()
+
+
+def test_initial_condition_solvers(oracle, dev):
+    """root_find_all (discrete Newton on IC_cons from the reference's start
+    r = omega0^(-2/3), pphi = sqrt(r) at f = 20 Hz, eob/seobnrv4phm.py:1966-2009),
+    root_find_scalar_all (false position on IC_diss in [-3e-2, 0]) and the IC_diss
+    kernel itself, against the oracle's restatements."""
+    n = 32
+    p = configs.uneven_batch(n=n, seed=17)
+    mt = p["m1"] + p["m2"]
+    omega0 = 20.0 * (mt * P.MTSUN_SI * np.pi)
+    add = oracle.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"], omega0=omega0)
+    co = oracle.fac_coeffs(add)
+    rg = omega0 ** (-2.0 / 3)
+    x0 = np.array([rg, np.sqrt(rg)])
+    x_r = oracle.root_find_all(x0, add, 1000, 1e-12)
+    xo = torch.zeros(2 * n, dtype=torch.float64, device=dev)
+    pyEOB.root_find_all(xo, _T(x0.reshape(-1), dev), torch.zeros(4 * n, dtype=torch.float64, device=dev), _T(add.reshape(-1), dev),
+                        1000, 1e-12, n, 2, 4, 80)
+    x = xo.cpu().numpy().reshape(2, n)
+    np.testing.assert_allclose(x, x_r, rtol=1e-10)
+    # the root: dH/dr = 0 and dH/dpphi = omega0 at pr = 0
+    g = oracle.grad_H(np.array([x[0], np.zeros(n), np.zeros(n), x[1]]), add)
+    assert np.all(np.abs(g[0]) < 1e-12 * np.abs(g[3])) and np.allclose(g[3], omega0, rtol=1e-10)
+
+    ode = _ode(add, dev)
+    args = np.array([x_r[0], np.zeros(n), np.zeros(n), x_r[1]])
+    bounds = np.tile(np.array([-3e-2, 0.0]), (n, 1)).T.copy()
+    pr_r = oracle.root_find_scalar_all(bounds, args, add, co, 1000, 1e-12)
+    pr = torch.zeros(n, dtype=torch.float64, device=dev)
+    ode.root_find_scalar_all(pr, _T(bounds.reshape(-1), dev), _T(args.reshape(-1), dev), _T(add.reshape(-1), dev), 1000, 1e-12, n, 4, 80)
+    # the false position stops at |IC_diss| < 1e-12 (absolute), which fixes pr to about 1e-8 relative
+    np.testing.assert_allclose(pr.cpu().numpy(), pr_r, rtol=1e-8, atol=1e-13)
+
+    rng = np.random.default_rng(18)
+    prs = rng.uniform(-3e-2, -1e-3, n)
+    out_r, args_r, add_r, g_r, gt_r, hs_r, F_r = oracle.ic_diss(prs, args, add, co)
+    ta, tadd = _T(args, dev), _T(add, dev)
+    out = torch.zeros(n, dtype=torch.float64, device=dev)
+    gd = torch.zeros((4, n), dtype=torch.float64, device=dev)
+    gt = torch.zeros((4, n), dtype=torch.float64, device=dev)
+    hs = torch.zeros((16, n), dtype=torch.float64, device=dev)
+    F = torch.zeros((2, n), dtype=torch.float64, device=dev)
+    ode.IC_diss(out, _T(prs, dev), ta, tadd, gd, gt, hs, F, n)
+    np.testing.assert_allclose(out.cpu().numpy(), out_r, rtol=1e-8)
+    np.testing.assert_array_equal(ta.cpu().numpy(), args_r)                         # pr written into args row 2
+    np.testing.assert_allclose(tadd.cpu().numpy()[4], add_r[4], rtol=2e-10)         # omega row := dH/dpphi
+    np.testing.assert_allclose(gd.cpu().numpy(), g_r, rtol=2e-10, atol=1e-16)
+    np.testing.assert_allclose(hs.cpu().numpy(), hs_r, rtol=3e-9, atol=1e-14)
+    np.testing.assert_allclose(F.cpu().numpy(), F_r, rtol=5e-10, atol=1e-20)
+    np.testing.assert_allclose(gt.cpu().numpy()[[0, 3]], gt_r[[0, 3]], rtol=2e-10, atol=1e-16)
+    ode.deallocate_information()

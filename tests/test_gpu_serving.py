@@ -90,3 +90,39 @@ def test_admission_waits_for_slots(dev):
     assert srv.pool.free == [(0, 8)]
     with pytest.raises(ValueError):
         srv.run([_req(9, 9, 400, 0.0)])
+
+
+def test_footprint_classes_and_reruns(dev):
+    """Size classes: each binary runs in the smallest class its projected length
+    fits; with a projection far too short the binaries fill their rows, run
+    again in larger classes, and every result still equals the batch
+    pipeline's (bit for bit)."""
+    reqs = []
+    for i, (nb, seed) in enumerate(((5, 501), (7, 502), (3, 503))):
+        p = configs.small_batch(n=nb, seed=seed)
+        p["r0"] = np.linspace(10.0, 16.0, nb)      # 231-332 stored points each
+        reqs.append(Request(i, 0.0005 * i, p))
+    data = _data()
+    refs = {}
+    for r in reqs:
+        p = r.params
+        b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=4096)
+        b.initial_conditions(p["r0"])
+        b.integrate(rtol=1e-10, atol=1e-10, two_pass=True)
+        b.hlm22_knots()
+        b.build_splines()
+        ht = P.rfft_batched(b.resample_hplus(p["m1"] + p["m2"], DT, N))
+        refs[r.rid] = P.fd_likelihood(ht, torch.as_tensor(data, dtype=torch.complex128, device=dev), 1.0 / (N * DT)).cpu().numpy()
+    classes = [(256, 16), (1024, 16), (4096, 16)]
+    for rpl, expect_reruns in ((11.0, False), (0.3, True)):
+        srv = Server(data, df=1.0 / (N * DT), dt=DT, N=N, classes=classes, streams=4, rows_per_length=rpl)
+        parts = [srv.split(r) for r in reqs]
+        res = srv.run(reqs)
+        assert sorted(x.rid for x in res) == [0, 1, 2]
+        for x in res:
+            np.testing.assert_array_equal(x.logl, refs[x.rid])
+        assert (srv.reruns > 0) == expect_reruns, srv.reruns
+        for pool in srv.pools:
+            assert pool.free == [(0, pool.capacity)]
+        if not expect_reruns:
+            assert len({pt.cls for ps in parts for pt in ps}) >= 1

@@ -95,3 +95,22 @@ def test_bench_refuses_several_gpus():
                          timeout=120)
     assert res.returncode == 2
     assert "one GPU" in res.stderr
+
+
+def test_projection_classes_and_budget():
+    from eobgpu_amd.serving import assign_classes, classes_for_budget, projected_rows
+    p = configs.uneven_batch(n=64, seed=2)
+    rows = projected_rows(p)
+    assert rows.shape == (64,) and np.all(rows > 64)
+    k = assign_classes(rows, [512, 1024, 2048, 4096])
+    cr = np.array([512, 1024, 2048, 4096])
+    assert np.all((cr[k] >= rows) | (k == 3))             # the class covers the projection, or it is the largest
+    assert np.all((k == 0) | (cr[np.maximum(k - 1, 0)] < rows))   # and no smaller class would
+    assert list(assign_classes([1, 512, 513, 10**6], [512, 1024])) == [0, 0, 1, 1]
+    import eobgpu_amd._lib as L
+    try:
+        L.lib()
+    except Exception:
+        pytest.skip("library not built")
+    cls = classes_for_budget(8 << 30, 1 << 12, class_rows=(512, 2048))
+    assert [ms for ms, _ in cls] == [512, 2048] and cls[0][1] > cls[1][1] > 0

@@ -462,3 +462,32 @@ def test_c_abi_tableau_matches_oracle(oracle):
     out = np.zeros(183)
     assert _lib.lib().eobhip_dop853_tableau(out.ctypes.data_as(ctypes.c_void_p)) == 0
     np.testing.assert_array_equal(out, oracle.tableau())
+
+
+# -------------------------------------------------------- detector projection
+def test_detector_response_closed_form(oracle):
+    """An L-shaped detector with arms along x and y, D = (xx - yy)/2, at the
+    origin: with this wave frame (m = -u sin psi - v cos psi, n = -u cos psi +
+    v sin psi) the antenna patterns are F+ = -1/2 (1 + cos^2 th) cos 2ph cos 2psi -
+    cos th sin 2ph sin 2psi and Fx = 1/2 (1 + cos^2 th) cos 2ph sin 2psi -
+    cos th sin 2ph cos 2psi; a vertex at distance c along the source direction
+    gives a delay of 1 s; F+^2 + Fx^2 does not depend on psi."""
+    rng = np.random.default_rng(5)
+    n = 50
+    ra, dec, psi = rng.uniform(0, 2 * np.pi, n), rng.uniform(-1.5, 1.5, n), rng.uniform(0, np.pi, n)
+    gmst = rng.uniform(0, 2 * np.pi, n)
+    D = np.zeros((2, 3, 3))
+    D[0, 0, 0], D[0, 1, 1] = 0.5, -0.5
+    D[1] = D[0]
+    c = 299792458.0
+    V = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, c]])
+    Fp, Fc, dt = oracle.detector_information(ra, dec, gmst, psi, D, V)
+    th, ph = np.pi / 2 - dec, ra - gmst
+    fp = -0.5 * (1 + np.cos(th) ** 2) * np.cos(2 * ph) * np.cos(2 * psi) - np.cos(th) * np.sin(2 * ph) * np.sin(2 * psi)
+    fc = 0.5 * (1 + np.cos(th) ** 2) * np.cos(2 * ph) * np.sin(2 * psi) - np.cos(th) * np.sin(2 * ph) * np.cos(2 * psi)
+    np.testing.assert_allclose(Fp[0], fp, atol=1e-14)
+    np.testing.assert_allclose(Fc[0], fc, atol=1e-14)
+    np.testing.assert_allclose(dt[0], 0.0, atol=0)
+    np.testing.assert_allclose(dt[1], np.cos(th), atol=1e-15)
+    Fp2, Fc2, _ = oracle.detector_information(ra, dec, gmst, psi + 0.7, D, V)
+    np.testing.assert_allclose(Fp2 ** 2 + Fc2 ** 2, Fp ** 2 + Fc ** 2, atol=1e-14)

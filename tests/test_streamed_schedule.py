@@ -70,3 +70,25 @@ def test_schedule_raises_when_flags_are_missing():
     with pytest.raises(RuntimeError, match="without flagging"):
         drain_finished(sim.done, 2, sim.launch, sim.complete, sim.is_ready, sim.ended, poll_s=0.0)
     assert len(np.concatenate([h["idx"] for h in sim.launched])) == 20
+
+
+@pytest.mark.parametrize("max_group", [1, 5, 64])
+def test_schedule_group_limit(max_group):
+    """Groups never exceed max_group binaries and still cover every binary once."""
+    sim = Sim(300, 3, 7)
+    sim.tick = _burst_tick(sim)
+    groups = drain_finished(sim.done, 3, sim.launch, sim.complete, sim.is_ready, sim.ended, poll_s=0.0, max_group=max_group)
+    assert sorted(np.concatenate(groups).tolist()) == list(range(300))
+    assert max(len(g) for g in groups) <= max_group
+    assert sim.completed == list(range(len(sim.launched)))
+
+
+def _burst_tick(sim):
+    """An integrator that finishes binaries in bursts of up to 40 (more than a group holds)."""
+    def tick():
+        sim.clock += 1
+        for _ in range(int(sim.rng.integers(0, 41))):
+            if sim.next < sim.done.size:
+                sim.done[sim.order[sim.next]] = 1
+                sim.next += 1
+    return tick
