@@ -86,7 +86,10 @@ _SIGS.update({
     "eobhip_rfft_batched": [c_vp, c_vp, c_int, c_int, c_vp],
     "eobhip_fd_likelihood": [c_vp, c_vp, c_vp, c_int, c_int, c_double, c_double, c_vp, c_vp],
     "eobhip_fd_likelihood_partial_doubles": [c_int, c_int],
+    "eobhip_fd_likelihood_static_max_partials": [],
+    "eobhip_fd_likelihood_static": [c_vp, c_vp, c_int, c_int, c_double, c_double, c_vp, c_vp],
 })
+_RESTYPES["eobhip_fd_likelihood_static_max_partials"] = c_size
 _RESTYPES["eobhip_fd_likelihood_partial_doubles"] = c_size
 _RESTYPES["eobhip_spline_scratch_doubles"] = c_size
 _SIGS.update({

@@ -540,3 +540,56 @@ size_t eobhip_fd_likelihood_partial_doubles(int n, int F) {
 
 }  // extern "C"
 
+// ---------------------------------------------- likelihood without allocation
+// The partial sums of eobhip_fd_likelihood_static live in file-scope device
+// memory: one copy per GPU, held until the process ends.  Host code reaches a
+// GPU's copy only through the address hipGetSymbolAddress returns with that GPU
+// current (the array's name is a host-side stand-in, not a device address).
+namespace fdlik_static {
+constexpr size_t kScratchDoubles = size_t(1) << 19;   // 4 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_lock;
+static double* g_ptr[kMaxDevices] = {};
+
+// this GPU's copy of `scratch`, looked up on the first call on the device
+static int scratch_ptr(double** out) {
+    int dev = -1;
+    int rc = hip_status(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("fd_likelihood_static: device %d outside the %d devices the library tracks", dev, kMaxDevices);
+        return EOBHIP_EINVAL;
+    }
+    std::lock_guard<std::mutex> g(g_lock);
+    if (!g_ptr[dev]) {
+        void* p = nullptr;
+        rc = hip_status(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        g_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_ptr[dev];
+    return EOBHIP_OK;
+}
+}  // namespace fdlik_static
+
+extern "C" {
+
+size_t eobhip_fd_likelihood_static_max_partials(void) { return fdlik_static::kScratchDoubles; }
+
+int eobhip_fd_likelihood_static(const double* htilde, const double* dtilde, int n, int F, double df, double f0, double* logl,
+                                void* stream) {
+    EOBHIP_REQUIRE(n >= 1 && F >= 1, "bad sizes");
+    const size_t need = eobhip_fd_likelihood_partial_doubles(n, F);
+    EOBHIP_REQUIRE(need <= fdlik_static::kScratchDoubles,
+                   "fd_likelihood_static: %zu partial sums (n = %d, F = %d) exceed the library's static buffer of %zu; "
+                   "use eobhip_fd_likelihood with a caller-provided buffer",
+                   need, n, F, fdlik_static::kScratchDoubles);
+    double* partial = nullptr;
+    const int rc = fdlik_static::scratch_ptr(&partial);
+    if (rc) return rc;
+    // every (binary, chunk) partial is written before it is read: nothing to clear
+    return eobhip_fd_likelihood(htilde, dtilde, partial, n, F, df, f0, logl, stream);
+}
+
+}  // extern "C"

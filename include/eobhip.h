@@ -275,6 +275,16 @@ int eobhip_rfft_batched(const double* in, double* out, int n, int N, void* strea
 int eobhip_fd_likelihood(const double* htilde, const double* dtilde, double* partial, int n, int F, double df, double f0,
                          double* logl, void* stream);
 size_t eobhip_fd_likelihood_partial_doubles(int n, int F);
+/* eobhip_fd_likelihood without a caller-provided buffer and without allocating:
+ * the partial sums go to a static device array of the library (4 MiB per GPU,
+ * eobhip_fd_likelihood_static_max_partials() doubles, held until the process
+ * ends), so n * ceil(F / 2048) may not exceed that count (EOBHIP_EINVAL
+ * otherwise).  Calls that overlap on one GPU share the array and race.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops). */
+size_t eobhip_fd_likelihood_static_max_partials(void);
+int eobhip_fd_likelihood_static(const double* htilde, const double* dtilde, int n, int F, double df, double f0, double* logl,
+                                void* stream);
 
 /* ------------------------------------------------ likelihood from the support
  * The same log L as eobhip_fd_likelihood on rfft(h+) (the reference's rfft +
