@@ -67,6 +67,11 @@ def parse():
                          "work per group of finished binaries), chunked (ChunkedPass) or the one-batch pass")
     ap.add_argument("--chunk-fractions", default="0.1,0.2,0.7",
                     help="ChunkedPass shares (longest binaries first), timed beside the headline pass; '1' skips it")
+    ap.add_argument("--sustained", type=int, default=4,
+                    help="batches of n fresh binaries integrated back to back in one streamed pass (a SIMD freed by a "
+                         "short binary starts the next batch's binaries), reported beside the headline; 0 skips it")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip timing the pass captured as one HIP graph (pipeline.GraphPass, full-length likelihood)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
     ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
@@ -226,6 +231,78 @@ def time_streamed(pipe, steps, warmup, group=64, block=64, grid_groups=0):
         attempts.append(int(sp.batch.attempts.sum().item()))
     wall = (time.perf_counter() - t0) / steps
     return wall, logl_same, t_traj / steps, len(sp.groups), t_ing / steps * 1e3, attempts
+
+
+def time_sustained(pipe, batches, steps, warmup, group=64, block=256):
+    """Back-to-back batches: `batches` fresh sets of n binaries (seeds 2000 +
+    batches k + b) in one StreamedPass whose work queue takes them batch by
+    batch (each batch longest expected first), so that the SIMDs freed by a
+    batch's short binaries start the next batch's binaries.  Each binary's log L
+    is its single-batch value (binaries are independent).  One integrator
+    wavefront per SIMD (grid 1024 groups) and two (2048) are timed.  Returns
+    {grid_groups: (wall per pass, evals/s)} and the ingestion time."""
+    torch, P = pipe.torch, pipe.P
+    from eobgpu_amd import configs
+    n = pipe.n
+    sets = []
+    for k in range(warmup + steps):
+        ps = [configs.uneven_batch(n=n, seed=2000 + batches * k + j) for j in range(batches)]
+        cat = {key: np.concatenate([q[key] for q in ps]) for key in ("m1", "m2", "chi1", "chi2", "r0")}
+        order = np.concatenate([j * n + np.argsort(-P.expected_length(q["r0"], q["m1"], q["m2"], q["chi1"], q["chi2"]), kind="stable")
+                                for j, q in enumerate(ps)])
+        sets.append((cat, order))
+    cat, order = sets[0]
+    sp = P.StreamedPass(cat["m1"], cat["m2"], cat["chi1"], cat["chi2"], cat["r0"], pipe.lik, cat["m1"] + cat["m2"], pipe.dt,
+                        max_step=pipe.b.max_step, rtol=pipe.rtol, order=order)
+    out = {}
+    t_ing = 0.0
+    for grid in (1024, 2048):
+        for k in range(warmup):
+            c, o = sets[k]
+            sp.set_params(c["m1"], c["m2"], c["chi1"], c["chi2"], c["r0"], order=o)
+            sp.run(group=group, block=block, grid_groups=grid)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            c, o = sets[warmup + k]
+            ti = time.perf_counter()
+            sp.set_params(c["m1"], c["m2"], c["chi1"], c["chi2"], c["r0"], order=o)
+            t_ing += time.perf_counter() - ti
+            sp.run(group=group, block=block, grid_groups=grid)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        out[grid] = (wall, batches * n / wall)
+    return out, t_ing / (2 * steps) * 1e3
+
+
+def time_graph(pipe, steps, warmup):
+    """The one-batch pass with the full-length likelihood captured as one HIP graph
+    (pipeline.GraphPass): every timed pass takes a fresh batch (seeds 3000 + k)
+    through set_params and one graph launch; the same launches issued one by one
+    (run_eager) are timed beside it.  Returns ms per pass for both."""
+    torch, P = pipe.torch, pipe.P
+    from eobgpu_amd import configs
+    gp = P.GraphPass(pipe.n, pipe.data, pipe.N, pipe.dt, df=pipe.df, max_step=pipe.b.max_step, rtol=pipe.rtol)
+    props = [configs.uneven_batch(n=pipe.n, seed=3000 + k) for k in range(warmup + steps)]
+
+    def put(q):
+        gp.set_params(q["m1"], q["m2"], q["chi1"], q["chi2"], q["r0"], amp=pipe.amp)
+
+    put(props[0])
+    gp.capture()
+    out = {}
+    for label, fn in (("graph", gp.run), ("eager", gp.run_eager)):
+        for k in range(warmup):
+            put(props[k])
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            put(props[warmup + k])
+            fn()
+        torch.cuda.synchronize()
+        out[label] = (time.perf_counter() - t0) / steps * 1e3
+    return out
 
 
 def cpu_info():
@@ -521,15 +598,8 @@ def main():
 
     chunked = streamed = None
     headline_wall = wall
-    fractions = [float(x) for x in args.chunk_fractions.split(",") if x.strip()]
-    if len(fractions) > 1 or args.pass_kind == "chunked":
-        wall_c, logl_c, traj_c, sizes = time_chunked(pipe, fractions, args.steps, args.warmup, group=group)
-        chunked = {"fractions": fractions, "chunk_sizes": sizes, "ms_per_step": wall_c * 1e3,
-                   "trajectory_longest_chunk_ms": traj_c, "one_batch_ms_per_step": wall * 1e3,
-                   "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_c - logl_support) / np.abs(logl_support)))}
-        if args.pass_kind == "chunked":
-            headline_wall = wall_c
-        print(f"[bench] chunked pass: {json.dumps(chunked)}", file=sys.stderr, flush=True)
+    # the streamed passes before the chunked one: a ChunkedPass timed first left the
+    # streamed pass slower in the same process
     if args.pass_kind == "streamed":
         wall_s, logl_s, traj_s, ngroups, ingest_ms, att_s = time_streamed(pipe, args.steps, args.warmup, group=group, block=block,
                                                                          grid_groups=grid)
@@ -540,6 +610,33 @@ def main():
                     "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_s - logl_support) / np.abs(logl_support)))}
         headline_wall = wall_s
         print(f"[bench] streamed pass: {json.dumps(streamed)}", file=sys.stderr, flush=True)
+
+    sustained = None
+    if args.sustained > 0 and args.pass_kind == "streamed":
+        res, ing = time_sustained(pipe, args.sustained, max(1, min(args.steps, 3)), 1, group=group, block=block)
+        best = max(res, key=lambda g: res[g][1])
+        sustained = {"batches": args.sustained, "binaries_per_pass": args.sustained * args.n, "fresh_batches": True,
+                     "by_grid_groups": {str(g): {"ms_per_pass": w * 1e3, "evals_per_s": v} for g, (w, v) in res.items()},
+                     "evals_per_s": res[best][1], "grid_groups": best, "host_ingest_ms": ing,
+                     "single_batch_evals_per_s": args.n / headline_wall}
+        print(f"[bench] sustained ({args.sustained} batches back to back): {json.dumps(sustained)}", file=sys.stderr, flush=True)
+
+    fractions = [float(x) for x in args.chunk_fractions.split(",") if x.strip()]
+    if len(fractions) > 1 or args.pass_kind == "chunked":
+        wall_c, logl_c, traj_c, sizes = time_chunked(pipe, fractions, args.steps, args.warmup, group=group)
+        chunked = {"fractions": fractions, "chunk_sizes": sizes, "ms_per_step": wall_c * 1e3,
+                   "trajectory_longest_chunk_ms": traj_c, "one_batch_ms_per_step": wall * 1e3,
+                   "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_c - logl_support) / np.abs(logl_support)))}
+        if args.pass_kind == "chunked":
+            headline_wall = wall_c
+        print(f"[bench] chunked pass: {json.dumps(chunked)}", file=sys.stderr, flush=True)
+
+    graph = None
+    if not args.no_graph:
+        g = time_graph(pipe, max(1, min(args.steps, 3)), 1)
+        graph = {"ms_per_pass_graph": g["graph"], "ms_per_pass_eager": g["eager"], "fresh_batches": True,
+                 "likelihood": "full length (N-point transforms)", "one_batch_full_length_ms": full_length["ms_per_step"]}
+        print(f"[bench] graph-captured pass: {json.dumps(graph)}", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:
@@ -601,7 +698,9 @@ def main():
         "stages_ms": stages, "one_batch_ms_per_step": wall * 1e3, "streamed": streamed, "chunked": chunked, "steps_per_binary": {"min": int(ns.min()), "median": float(np.median(ns)), "max": int(ns.max())},
         "attempted_steps_total": int(total_attempts), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
         "logl_finite": bool(np.all(np.isfinite(logl))),
-        "full_length_likelihood": full_length, "plain_batch": plain, "served": served, "roofline": roof, "cpu_baseline": cpu,
+        "full_length_likelihood": full_length, "sustained": sustained, "graph_pass": graph, "plain_batch": plain, "served": served,
+        "roofline": roof,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

@@ -66,12 +66,16 @@ _SIGS = {
                                        c_int, c_int, c_vp, c_vp, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_ll_p,
                                        c_vp],
     "eobhip_hlm22_knots": [c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp, c_vp],
+    "eobhip_wave_mode_bytes": [],
+    "eobhip_wave_tables": [c_vp, c_int, c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_hlm_modes_knots": [c_vp, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp],
 }
 _RESTYPES = {
     "eobhip_last_error": ctypes.c_char_p,
     "eobhip_bindata_bytes": c_size,
     "eobhip_modes_bytes": c_size,
     "eobhip_integrate_scratch_bytes": c_size,
+    "eobhip_wave_mode_bytes": c_size,
 }
 # optional groups: present once the corresponding sources are built
 _SIGS.update({
@@ -81,6 +85,10 @@ _SIGS.update({
     "eobhip_spline_scratch_doubles": [c_int, c_int],
     "eobhip_resample_hplus": [c_vp] * 5 + [c_int, c_int, c_int, c_vp, c_vp, c_vp],
     "eobhip_resample_hpc": [c_vp] * 5 + [c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_spline_series_scratch_doubles": [c_int, c_int, c_int],
+    "eobhip_spline_build_series": [c_vp, c_vp, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp],
+    "eobhip_resample_modes": [c_vp] * 5 + [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_TDInterp2": [c_vp] * 7 + [c_int, c_vp, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_int, c_vp],
     "eobhip_detector_response": [c_vp] * 4 + [c_int, c_vp, c_vp, c_int, c_vp, c_vp, c_vp, c_vp],
     "eobhip_detector_signal": [c_vp] * 5 + [c_int, c_int, c_int, c_double, c_int, c_vp, c_vp],
     "eobhip_rfft_batched": [c_vp, c_vp, c_int, c_int, c_vp],
@@ -92,6 +100,7 @@ _SIGS.update({
 _RESTYPES["eobhip_fd_likelihood_static_max_partials"] = c_size
 _RESTYPES["eobhip_fd_likelihood_partial_doubles"] = c_size
 _RESTYPES["eobhip_spline_scratch_doubles"] = c_size
+_RESTYPES["eobhip_spline_series_scratch_doubles"] = c_size
 _SIGS.update({
     "eobhip_lik_bank_doubles": [c_int, c_int],
     "eobhip_lik_setup_scratch_bytes": [c_int],

@@ -11,6 +11,7 @@
 
 #include "eobhip_common.hpp"
 #include "eob_integrator.hpp"
+#include "eob_wave.hpp"
 
 using namespace eob;
 
@@ -28,6 +29,7 @@ using namespace eobhip_detail;
 struct eobhip_seobnr {
     ModeCoef* modes = nullptr;   // [numSys][35], prefix-independent (N2 set per call)
     WaveCoef* wave = nullptr;    // [numSys]
+    WaveMode* wmodes = nullptr;  // [numSys][35]: the waveform's tables (EOBGetSpinFactorizedWaveform)
     const double* prefixes_hlms = nullptr;
     int numSys = 0;
     bool allocated = false;
@@ -312,21 +314,40 @@ __global__ __launch_bounds__(kBlock) void k_update_info(ModeCoef* modes, WaveCoe
     }
 }
 
-// EOBGetSpinFactorizedWaveform_kernel for (2,2): one block per binary, threads over knots
-__global__ __launch_bounds__(kBlock) void k_hlm22_dropin(double* out, const double* r, const double* phi, const double* v,
-                                                         const double* Hreal, const double* eta, const double* vPhiIn,
-                                                         const double* prefix, double h22c, int numSys, int L,
-                                                         const ModeCoef* modes, const WaveCoef* wave, const int* idx) {
+// The waveform's per-mode tables of every binary (one thread per binary and mode),
+// from the flux tables k_update_info built and the same inputs.
+__global__ __launch_bounds__(kBlock) void k_wave_tables(WaveMode* wm, const ModeCoef* modes, const WaveCoef* wave, const double* m1,
+                                                        const double* m2, const double* eta, const double* chiS, const double* chiA,
+                                                        int use_hm, int n) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * kNumModes) return;
+    const int i = (int)(t / kNumModes), q = (int)(t % kNumModes);
+    int l, m;
+    mode_lm(q, l, m);
+    wave_mode_setup(wm[t], modes[t], l, m, m1[i], m2[i], eta[i], chiS[i], chiA[i], use_hm, wave + i);
+}
+
+// EOBGetSpinFactorizedWaveform_kernel for any (l, m): one block per binary,
+// threads over knots; h22_calib replaces the (2,2) table's v^12 coefficient
+__global__ __launch_bounds__(kBlock) void k_hlm_dropin(double* out, const double* r, const double* phi, const double* pphi,
+                                                       const double* v, const double* Hreal, const double* eta, const double* vPhiIn,
+                                                       const double* prefix, double h22c, int numSys, int L, const WaveMode* wmodes,
+                                                       const int* idx, int fi, int nsys) {
     for (int b = blockIdx.x; b < numSys; b += gridDim.x) {
         const int hi = idx[b];
-        ModeCoef z = modes[(size_t)hi * kNumModes + 1];
-        z.c[10] = h22c;
-        WaveCoef w = wave[hi];
-        w.pf22_re = prefix[2 * b];
-        w.pf22_im = prefix[2 * b + 1];
+        if (hi < 0 || hi >= nsys) {   // an index outside the handle's binaries: NaN, not a stray read
+            for (int s = threadIdx.x; s < L; s += blockDim.x) {
+                out[2 * ((size_t)b * L + s)] = NAN;
+                out[2 * ((size_t)b * L + s) + 1] = NAN;
+            }
+            continue;
+        }
+        WaveMode w = wmodes[(size_t)hi * kNumModes + fi];
+        if (w.l == 2 && w.m == 2) w.c[10] = h22c;
+        const double pre_re = prefix[2 * b], pre_im = prefix[2 * b + 1];
         for (int s = threadIdx.x; s < L; s += blockDim.x) {
             const size_t q = (size_t)b * L + s;
-            const Cplx h = hlm22_knot(z, w, eta[b], r[q], phi[q], v[q], Hreal[q], vPhiIn[q]);
+            const Cplx h = hlm_knot(w, pre_re, pre_im, eta[b], r[q], phi[q], pphi[q], v[q], Hreal[q], vPhiIn[q]);
             out[2 * q] = h.re;
             out[2 * q + 1] = h.im;
         }
@@ -1061,6 +1082,61 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
+// ----------------------------------------------------------- higher modes
+// The modes a batch evaluates on its knots (get_hlms' ells, mms).
+constexpr int kMaxWaveModes = 15;
+struct ModeList {
+    int nm;
+    int l[kMaxWaveModes], m[kMaxWaveModes];
+};
+
+// Waveform tables of the listed modes for every binary of a batch (one thread
+// per binary and mode), from its additional args (masses and spins, M = 1) and
+// the flux tables of its setup: wtab [n][nm].
+__global__ __launch_bounds__(kBlock) void k_wave_tables_batch(const double* add, int n, int use_hm, const ModeCoef* modes,
+                                                              const BinData* bd, ModeList ml, WaveMode* wtab) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n * ml.nm) return;
+    const int i = (int)(t / ml.nm), j = (int)(t % ml.nm);
+    const int l = ml.l[j], m = ml.m[j];
+    const double m1 = add[i], m2 = add[n + i], c1 = add[2 * n + i], c2 = add[3 * n + i];
+    const double eta = m1 * m2 / ((m1 + m2) * (m1 + m2));
+    wave_mode_setup(wtab[t], modes[(size_t)i * kNumModes + flux_index(l, m)], l, m, m1, m2, eta, 0.5 * (c1 + c2), 0.5 * (c1 - c2), use_hm,
+                    &bd[i].w);
+}
+
+// The listed modes on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221)
+// and the series compute_full_waveform splines (:2336-2356).  knots
+// [n][max_step][2 nm + 1] = Re(h_j e^{i m_j phi}) for j < nm, their Im, phi.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_hlm_modes_knots(
+    const double* dense, const int* num_steps, const BinData* bdp, const WaveMode* wtab, const double* add, int n, int max_step, int nm,
+    double* knots) {
+    const int b = blockIdx.y;
+    if (b >= n) return;
+    const int len = num_steps[b];
+    const BinData& bd = bdp[b];
+    const int S = 2 * nm + 1;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < len; s += gridDim.x * blockDim.x) {
+        const double* row = dense + ((size_t)b * max_step + s) * 5;
+        const double r = row[1], phi = row[2], pr = row[3], L = row[4];
+        const Dual dfull = ham_tangent(bd.h, r, pr, L, 2);
+        const double omega = dfull.d;
+        const double omc = ham_tangent(bd.h, r, pr, L, 3).d;
+        const double vPhiIn = 1.0 / (omc * omc * (r * r * r));
+        const double Hreal = bd.nu * dfull.v;
+        const double v = cbrt(omega);
+        double* o = knots + ((size_t)b * max_step + s) * S;
+        for (int j = 0; j < nm; ++j) {
+            const WaveMode& w = wtab[(size_t)b * nm + j];
+            const int fi = flux_index(w.l, w.m);
+            const Cplx h = hlm_knot(w, add[(10 + 2 * fi) * n + b], add[(11 + 2 * fi) * n + b], bd.nu, r, phi, L, v, Hreal, vPhiIn, true);
+            o[j] = h.re;
+            o[nm + j] = h.im;
+        }
+        o[2 * nm] = phi;
+    }
+}
+
 template <int G>
 int launch_persistent(const IntegParams& P, int block, int grid_groups, hipStream_t st) {
     const int groups_per_block = block / G;
@@ -1192,8 +1268,10 @@ int eobhip_seobnr_deallocate_information(eobhip_seobnr_t h) {
     if (h->allocated) {
         hipFree(h->modes);
         hipFree(h->wave);
+        hipFree(h->wmodes);
         h->modes = nullptr;
         h->wave = nullptr;
+        h->wmodes = nullptr;
         h->allocated = false;
         h->numSys = 0;
     }
@@ -1217,12 +1295,19 @@ int eobhip_seobnr_update_information(eobhip_seobnr_t h, const double* m1, const 
     if (rc) return rc;
     rc = hip_status(hipMalloc(&h->wave, sizeof(WaveCoef) * (size_t)numSys), "hipMalloc wave");
     if (rc) { hipFree(h->modes); h->modes = nullptr; return rc; }
+    rc = hip_status(hipMalloc(&h->wmodes, sizeof(WaveMode) * (size_t)numSys * kNumModes), "hipMalloc wmodes");
+    if (rc) { hipFree(h->modes); hipFree(h->wave); h->modes = nullptr; h->wave = nullptr; return rc; }
     h->numSys = numSys;
     h->prefixes_hlms = newtonian_prefixes_hlms;
     h->allocated = true;
     hipLaunchKernelGGL(k_update_info, dim3(blocks_for(numSys, kBlock)), dim3(kBlock), 0, as_stream(stream), h->modes, h->wave, m1, m2,
                        eta, tplspin, chi_S, chi_A, use_hm, numSys);
-    return check_launch("update_information");
+    rc = check_launch("update_information");
+    if (rc) return rc;
+    const long long nw = (long long)numSys * kNumModes;
+    hipLaunchKernelGGL(k_wave_tables, dim3((unsigned)((nw + kBlock - 1) / kBlock)), dim3(kBlock), 0, as_stream(stream), h->wmodes, h->modes,
+                       h->wave, m1, m2, eta, chi_S, chi_A, use_hm, numSys);
+    return check_launch("update_information (waveform tables)");
 }
 
 int eobhip_seobnr_RR_force(eobhip_seobnr_t h, double* force_out, double* grad_out, const double* args, const double* add, int numSys,
@@ -1270,15 +1355,17 @@ int eobhip_seobnr_EOBGetSpinFactorizedWaveform(eobhip_seobnr_t h, double* out, c
                                                const double* pphi, const double* v, const double* Hreal, const double* eta,
                                                const double* vPhiInput, int l, int m, const double* newtonian_prefix, double h22_calib,
                                                int numSys, int traj_length, const int* idx, void* stream) {
-    (void)pr; (void)pphi;
+    (void)pr;
     EOBHIP_REQUIRE(h && h->allocated, "EOBGetSpinFactorizedWaveform: update_information has not been called");
-    if (!(l == 2 && m == 2)) {
-        set_error("EOBGetSpinFactorizedWaveform: only (l, m) = (2, 2) is implemented, got (%d, %d)", l, m);
+    if (!(l >= 2 && l <= 8 && m >= 1 && m <= l)) {
+        set_error("EOBGetSpinFactorizedWaveform: (l, m) = (%d, %d) outside 2 <= l <= 8, 1 <= m <= l", l, m);
         return EOBHIP_EUNSUPPORTED;
     }
+    EOBHIP_REQUIRE(numSys >= 0 && traj_length >= 0, "bad sizes");
     if (numSys == 0 || traj_length == 0) return EOBHIP_OK;
-    hipLaunchKernelGGL(k_hlm22_dropin, dim3(numSys), dim3(kBlock), 0, as_stream(stream), out, r, phi, v, Hreal, eta, vPhiInput,
-                       newtonian_prefix, h22_calib, numSys, traj_length, h->modes, h->wave, idx);
+    const int blocks = numSys < 65535 ? numSys : 65535;
+    hipLaunchKernelGGL(k_hlm_dropin, dim3(blocks), dim3(kBlock), 0, as_stream(stream), out, r, phi, pphi, v, Hreal, eta, vPhiInput,
+                       newtonian_prefix, h22_calib, numSys, traj_length, h->wmodes, idx, flux_index(l, m), h->numSys);
     return check_launch("EOBGetSpinFactorizedWaveform");
 }
 
@@ -1558,6 +1645,46 @@ int eobhip_integrate_batch_coupled(const double* condBound, const void* bindata,
     return integrate_impl(condBound, bindata, modes, n, max_step, rtol, atol, hInit, step_back, fine_step, tmax, two_pass, fix_step, x0,
                           step0, use_stop, 2, group, kBlock, 0, dense, num_steps, stop_info, pass1_steps, attempts, scratch,
                           iterations_out, nullptr, old_omegas, stream);
+}
+
+size_t eobhip_wave_mode_bytes(void) { return sizeof(WaveMode); }
+
+static int mode_list(int nm, const int* ls, const int* ms, ModeList& ml) {
+    EOBHIP_REQUIRE(nm >= 1 && nm <= kMaxWaveModes, "nm must be 1 .. 15");
+    ml.nm = nm;
+    for (int j = 0; j < nm; ++j) {
+        if (!(ls[j] >= 2 && ls[j] <= 8 && ms[j] >= 1 && ms[j] <= ls[j])) {
+            set_error("mode %d: (l, m) = (%d, %d) outside 2 <= l <= 8, 1 <= m <= l", j, ls[j], ms[j]);
+            return EOBHIP_EUNSUPPORTED;
+        }
+        ml.l[j] = ls[j];
+        ml.m[j] = ms[j];
+    }
+    return EOBHIP_OK;
+}
+
+int eobhip_wave_tables(const double* add, int n, int use_hm, const void* bindata, const void* modes, int nm, const int* ls,
+                       const int* ms, void* wtab, void* stream) {
+    EOBHIP_REQUIRE(n >= 0, "bad sizes");
+    ModeList ml;
+    const int rc = mode_list(nm, ls, ms, ml);
+    if (rc) return rc;
+    if (n == 0) return EOBHIP_OK;
+    const long long nt = (long long)n * nm;
+    hipLaunchKernelGGL(k_wave_tables_batch, dim3((unsigned)((nt + kBlock - 1) / kBlock)), dim3(kBlock), 0, as_stream(stream), add, n,
+                       use_hm, static_cast<const ModeCoef*>(modes), static_cast<const BinData*>(bindata), ml, static_cast<WaveMode*>(wtab));
+    return check_launch("wave_tables");
+}
+
+int eobhip_hlm_modes_knots(const double* dense, const int* num_steps, const void* bindata, const void* wtab, const double* add, int n,
+                           int max_step, int nm, double* knots, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && max_step >= 1 && nm >= 1 && nm <= kMaxWaveModes, "bad arguments");
+    if (n == 0) return EOBHIP_OK;
+    EOBHIP_REQUIRE(n <= 65535, "n must be <= 65535 (grid y dimension)");
+    const int bx = (max_step + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_hlm_modes_knots, dim3(bx < 64 ? bx : 64, n), dim3(kBlock), 0, as_stream(stream), dense, num_steps,
+                       static_cast<const BinData*>(bindata), static_cast<const WaveMode*>(wtab), add, n, max_step, nm, knots);
+    return check_launch("hlm_modes_knots");
 }
 
 int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bindata, const void* modes, int n, int max_step,

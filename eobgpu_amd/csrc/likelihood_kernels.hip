@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include "eobhip_common.hpp"
+#include "knot_window.hpp"
 
 using namespace eobhip_detail;
 
@@ -131,8 +132,8 @@ __global__ __launch_bounds__(kBlock) void k_lik_sizes(const double* __restrict__
 // h+ = amp Re[S(t) e^{-2 i phi(t)}] from the spline coefficients (the uniform
 // resample of the full-length path) on j < M_b, zero for nsamp <= j < M_b;
 // per-block partial sums of h_j g_j.  grid (gx, n); a block covers kSpan
-// consecutive samples, brackets their knot interval once, and each thread
-// searches inside the bracket.
+// consecutive samples, stages the knot times of their intervals in LDS
+// (knot_window) and each thread searches there.
 __global__ __launch_bounds__(kBlock) void k_lik_resample(const double* __restrict__ dense, const double* __restrict__ coef,
                                                          const int* __restrict__ num_steps, const double* __restrict__ amp,
                                                          const double* __restrict__ dT_M, const int* __restrict__ nsamp,
@@ -145,27 +146,17 @@ __global__ __launch_bounds__(kBlock) void k_lik_resample(const double* __restric
     double* out = hbuf + rowoff[b];
     const long long nb = nsamp[b];
     double acc = 0.0;
-    __shared__ int bracket[2];
+    __shared__ KnotWindow<kBlock> win;
     if (j0 < M) {
         const int len = num_steps[b];
         const double* T = dense + (size_t)b * max_step * 5;
         const double* C = coef + (size_t)b * max_step * 12;
         const double dT = dT_M[b];
         const double t0 = T[0];
-        const long long jmax = (j0 + kSpan < nb ? j0 + kSpan : nb) - 1;   // last sample with a value in this block
-        if (j0 < nb && threadIdx.x < 2) {
-            const double tj = t0 + (double)(threadIdx.x == 0 ? j0 : jmax) * dT;
-            int lo = 0, hi = len;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (T[5 * mid] <= tj) lo = mid + 1;
-                else hi = mid;
-            }
-            int ind = lo - 1;
-            if (ind < 0) ind = 0;
-            bracket[threadIdx.x] = ind;
+        if (j0 < nb) {
+            const long long jmax = (j0 + kSpan < nb ? j0 + kSpan : nb) - 1;   // last sample with a value in this block
+            knot_window<kBlock>(win, T, 5, len, t0 + (double)j0 * dT, t0 + (double)jmax * dT);
         }
-        __syncthreads();
         const double a = amp[b];
 #pragma unroll 4
         for (int u = 0; u < kPerThread; ++u) {
@@ -174,13 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_lik_resample(const double* __restric
             double val = 0.0;
             if (j < nb) {
                 const double tj = t0 + (double)j * dT;
-                int lo = bracket[0] + 1, hi = bracket[1] + 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (T[5 * mid] <= tj) lo = mid + 1;
-                    else hi = mid;
-                }
-                const int ind = lo - 1;
+                const int ind = knot_interval<kBlock>(win, T, 5, len, tj);
                 const double x = tj - T[5 * ind];
                 const double x2 = x * x, x3 = x2 * x;
                 const double* c = C + 12 * ind;
@@ -194,9 +179,9 @@ __global__ __launch_bounds__(kBlock) void k_lik_resample(const double* __restric
             }
             out[j] = val;
         }
-    } else {
-        __syncthreads();   // every thread of the block reaches the reduction's barrier
     }
+    // every thread of the block reaches the reduction's barrier (the window's
+    // barriers are taken by all threads or by none: j0 and nb are per block)
     const double s = block_sum(acc);
     if (threadIdx.x == 0) cross_partial[(size_t)b * gridDim.x + blockIdx.x] = s;
 }

@@ -15,6 +15,7 @@
 #include <utility>
 
 #include "eobhip_common.hpp"
+#include "knot_window.hpp"
 
 using namespace eobhip_detail;
 
@@ -104,143 +105,202 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_TD(const double* __restr
 }
 
 // ---------------------------------------------------- batched EOB splines
-// Three interpolants per binary over its trajectory knots t_k (units of M):
-// q = 0 phi, 1 Re(h22 e^{2 i phi}), 2 Im(h22 e^{2 i phi}).  coef [n][max_step][12]
-// holds (y, c1, c2, c3) per interpolant.
-// The same splines with one workgroup per interpolant.  The slope system is
-// split into up to 256 chunks of >= kChunkMin rows; each thread solves its
-// chunk together with kHalo rows on either side as a system of its own (cut
-// off where the halo ends) and keeps the slopes of its own rows.  Inside the
-// system every row is diagonally dominant (2(dx1 + dx2) against dx1 + dx2),
-// so the effect of the cut on a row k rows away is damped by at least 2^-k:
-// with 64 halo rows it is below 1e-19 of the neighbouring slopes, i.e. the
-// chunked solve returns the slopes of the whole system to rounding.  The
-// not-a-knot end rows are never cut (the first and last chunks start and end
-// there).  The coefficients are then formed by all threads in parallel.
+// nser interpolants per binary over its trajectory knots t_k (units of M):
+// knots [n][max_step][nser] (for the (2,2) pipeline nser = 3: phi, Re and Im of
+// h22 e^{2 i phi}), coef [n][max_step][4 nser] = (y, c1, c2, c3) per interpolant.
+//
+// One wavefront per binary builds all of them.  Each interpolant's slope
+// system is split into P <= 64 / nser chunks of >= kChunkMin rows, one lane
+// per (interpolant, chunk); the lane solves its chunk together with kHalo rows on either side as
+// a system of its own (cut off where the halo ends) and keeps the slopes of its
+// own rows.  Inside the system every row is diagonally dominant (2(dx1 + dx2)
+// against dx1 + dx2), so the effect of the cut on a row k rows away is damped
+// by at least 2^-k: with 64 halo rows it is below 1e-19 of the neighbouring
+// slopes, i.e. the chunked solve returns the slopes of the whole system to
+// rounding.  The not-a-knot end rows are never cut.
+//
+// Nothing of the system goes to memory: the rows are formed from t and y as the
+// sweeps reach them, the forward elimination keeps only its carry (c', d') at
+// the start of every kSeg rows (a checkpoint, 16 bytes per kSeg rows), and the
+// back substitution walks the segments top down, recomputing each segment's
+// (c', d') in registers from its checkpoint and writing the polynomial
+// coefficients of each interval as soon as both end slopes are known.  Memory
+// traffic is t and y read once and the coefficients written once (the
+// recomputation and the halos re-read rows from the caches).
 constexpr int kHalo = 64;
 constexpr int kChunkMin = 128;
-constexpr int kSplineThreads = 256;
+constexpr int kSplineLanes = 64;
+constexpr int kSeg = 16;
+constexpr int kMaxSeries = 32;
 
-__host__ __device__ inline int spline_chunks(int len) {
+__host__ __device__ inline int spline_chunks(int len, int nser) {
+    const int pmax = kSplineLanes / nser;
     int P = len / kChunkMin;
-    return P < 1 ? 1 : (P > kSplineThreads ? kSplineThreads : P);
+    return P < 1 ? 1 : (P > pmax ? pmax : P);
 }
-// per-interpolant scratch in doubles: slopes, the system's rows (4 per knot)
-// and the chunks' (c', d') rows, chunk p at 2 p (m + 2 kHalo) for m =
-// ceil(len / P): 2 P (m + 2 kHalo) <= 2 (len + P) + 256 P <= 4 len + 512,
-// since P = 1 or P <= len / kChunkMin, and P <= 256
-__host__ __device__ inline long long spline_scratch_per_interp(int max_step) {
-    return 9LL * max_step + 512;
+// checkpoints per lane for a binary of len knots (segments of its chunk's sweep)
+__host__ __device__ inline int spline_segs(int len, int nser) {
+    const int P = spline_chunks(len, nser);
+    const int m = (len + P - 1) / P;
+    return (m + 2 * kHalo + kSeg - 1) / kSeg + 1;
 }
-// rows whose loads a thread issues together in the serial sweeps: the sweeps
-// would otherwise wait one memory round trip per row
-constexpr int kSweepBatch = 16;
+// doubles per binary: nser P lanes x 2 doubles x spline_segs for every len <=
+// max_step (nser P 2 segs <= 2 nser len / kSeg + 2 P nser (1 + 2 kHalo) / kSeg +
+// 4 P nser with P nser <= 64; checked for nser <= 32 and every len up to 70000)
+__host__ __device__ inline long long spline_scratch_per_bin(int max_step, int nser) {
+    return (2LL * nser * max_step + 2LL * kSplineLanes * (2 * kHalo + 2 * kSeg)) / kSeg + 4 * kSplineLanes + 64;
+}
 
-__global__ __launch_bounds__(kSplineThreads) void k_spline_build_par(const double* __restrict__ dense, const double* __restrict__ knots,
-                                                                     const int* __restrict__ num_steps, int n, int max_step,
-                                                                     double* coef, double* scratch) {
-    const int b = blockIdx.x / 3, q = blockIdx.x % 3;
+// t and y of rows start-2 .. start+kSeg+1, clamped into [0, len-1]; issued together
+__device__ __forceinline__ void seg_load(const double* __restrict__ T, const double* __restrict__ Y, int ys, int start, int len,
+                                         double* wt, double* wy) {
+#pragma unroll
+    for (int k = 0; k < kSeg + 4; ++k) {
+        int i = start - 2 + k;
+        i = i < 0 ? 0 : (i > len - 1 ? len - 1 : i);
+        wt[k] = T[5 * i];
+        wy[k] = Y[(size_t)ys * i];
+    }
+}
+
+// Forward elimination over rows start .. end-1 (end - start <= kSeg) of the
+// system cut to rows lo .. hi-1, from the carry (pc, pd) of the row below;
+// kKeep stores every row's (c', d').  Interior row i = start + j reads the
+// window at j+1, j+2, j+3 (rows i-1, i, i+1); the two end rows (rfirst, rlast)
+// are formed once by the caller and picked by value, so that every window
+// index stays a compile-time constant (a select between window entries would
+// turn into a computed address and push the window out of registers).
+template <bool kKeep>
+__device__ __forceinline__ void seg_forward(const double* wt, const double* wy, int start, int end, int lo, int hi, int len,
+                                            const Row& rfirst, const Row& rlast, double& pc, double& pd, double* pcs, double* pds) {
+#pragma unroll
+    for (int j = 0; j < kSeg; ++j) {
+        const int i = start + j;
+        if (i < end) {
+            Row r = nak_row(1, 3, wt[j + 1], wt[j + 2], wt[j + 3], wy[j + 1], wy[j + 2], wy[j + 3]);   // the interior form
+            if (i == 0) r = rfirst;
+            if (i == len - 1) r = rlast;
+            const double a = (i == lo) ? 0.0 : r.ld;        // cut below the halo
+            const double c = (i == hi - 1) ? 0.0 : r.ud;    // cut above the halo
+            const double den = r.dg - a * pc;
+            pc = c / den;
+            pd = (r.b - a * pd) / den;
+            if constexpr (kKeep) {
+                pcs[j] = pc;
+                pds[j] = pd;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSplineLanes) void k_spline_build_seg(const double* __restrict__ dense, const double* __restrict__ knots,
+                                                                   const int* __restrict__ num_steps, int n, int max_step, int nser,
+                                                                   double* __restrict__ coef, double* __restrict__ scratch) {
+    const int b = blockIdx.x;
     if (b >= n) return;
     const int len = num_steps[b];
+    const int lane = threadIdx.x;
+    const int cs = 4 * nser;   // coefficient row stride
     const double* T = dense + (size_t)b * max_step * 5;
-    const double* Y = knots + (size_t)b * max_step * 3 + q;
-    double* C = coef + (size_t)b * max_step * 12 + 4 * q;
-    double* S = scratch + ((size_t)b * 3 + q) * spline_scratch_per_interp(max_step);
     if (len < 3) {
-        for (int i = threadIdx.x; i < len; i += blockDim.x) {
-            C[12 * i] = Y[3 * i];
-            C[12 * i + 1] = C[12 * i + 2] = C[12 * i + 3] = 0.0;
+        if (lane < nser) {
+            const double* Y = knots + (size_t)b * max_step * nser + lane;
+            double* C = coef + (size_t)b * max_step * cs + 4 * lane;
+            for (int i = 0; i < len; ++i) {
+                C[(size_t)cs * i] = Y[(size_t)nser * i];
+                C[(size_t)cs * i + 1] = C[(size_t)cs * i + 2] = C[(size_t)cs * i + 3] = 0.0;
+            }
         }
         return;
     }
-    const int P = spline_chunks(len);
+    const int P = spline_chunks(len, nser);
+    if (lane >= nser * P) return;
+    const int q = lane / P, cidx = lane % P;
+    const double* Y = knots + (size_t)b * max_step * nser + q;
+    double* C = coef + (size_t)b * max_step * cs + 4 * q;
     const int m = (len + P - 1) / P;
-    const int p = threadIdx.x;
-    // every row of the system (sub-diagonal, diagonal, super-diagonal, right
-    // side), by all threads at once
-    double* R = S + max_step;
-    for (int i = p; i < len; i += blockDim.x) {
-        int i0, i1, i2;
-        if (i == len - 1) { i0 = len - 3; i1 = len - 2; i2 = len - 1; }
-        else if (i == 0) { i0 = 0; i1 = 1; i2 = 2; }
-        else { i0 = i - 1; i1 = i; i2 = i + 1; }
-        const Row r = nak_row(i, len, T[5 * i0], T[5 * i1], T[5 * i2], Y[3 * i0], Y[3 * i1], Y[3 * i2]);
-        R[4 * i] = r.ld;
-        R[4 * i + 1] = r.dg;
-        R[4 * i + 2] = r.ud;
-        R[4 * i + 3] = r.b;
-    }
-    __syncthreads();
-    if (p < P) {
-        const int s0 = p * m, e0 = (s0 + m < len) ? s0 + m : len;
-        if (s0 < e0) {
-            const int lo = (s0 - kHalo > 0) ? s0 - kHalo : 0;
-            const int hi = (e0 + kHalo < len) ? e0 + kHalo : len;
-            double* cw = S + 5 * (size_t)max_step + (size_t)p * 2 * (m + 2 * kHalo);   // c' and d' of rows lo..hi-1
-            double pc = 0.0, pd = 0.0;
-            for (int i0 = lo; i0 < hi; i0 += kSweepBatch) {
-                double rl[kSweepBatch], rd[kSweepBatch], ru[kSweepBatch], rb[kSweepBatch];
+    const int s0 = cidx * m, e0 = (s0 + m < len) ? s0 + m : len;
+    if (s0 >= e0) return;
+    const int lo = (s0 - kHalo > 0) ? s0 - kHalo : 0;
+    const int hi = (e0 + kHalo < len) ? e0 + kHalo : len;
+    const int nseg = (hi - lo + kSeg - 1) / kSeg;
+    double* ck = scratch + (size_t)b * spline_scratch_per_bin(max_step, nser) + (size_t)lane * 2 * spline_segs(len, nser);
+    const Row rfirst = nak_row(0, len, T[0], T[5], T[10], Y[0], Y[nser], Y[2 * nser]);
+    const Row rlast = nak_row(len - 1, len, T[5 * (len - 3)], T[5 * (len - 2)], T[5 * (len - 1)], Y[(size_t)nser * (len - 3)],
+                              Y[(size_t)nser * (len - 2)], Y[(size_t)nser * (len - 1)]);
+    // the next segment's rows (and checkpoint) are loaded while this one is
+    // eliminated: the sweeps are chains of dependent divisions, and without the
+    // prefetch each segment would wait a memory round trip for its rows
+    double wt[kSeg + 4], wy[kSeg + 4], nt[kSeg + 4], ny[kSeg + 4];
+    double pc = 0.0, pd = 0.0;
+    seg_load(T, Y, nser, lo, len, wt, wy);
+    for (int sg = 0; sg < nseg; ++sg) {
+        const int start = lo + sg * kSeg, end = (start + kSeg < hi) ? start + kSeg : hi;
+        ck[2 * sg] = pc;
+        ck[2 * sg + 1] = pd;
+        if (sg + 1 < nseg) seg_load(T, Y, nser, start + kSeg, len, nt, ny);
+        seg_forward<false>(wt, wy, start, end, lo, hi, len, rfirst, rlast, pc, pd, nullptr, nullptr);
+        if (sg + 1 < nseg) {
 #pragma unroll
-                for (int j = 0; j < kSweepBatch; ++j) {
-                    const int i = (i0 + j < hi) ? i0 + j : hi - 1;   // in bounds; rows past hi are not used
-                    rl[j] = R[4 * i]; rd[j] = R[4 * i + 1]; ru[j] = R[4 * i + 2]; rb[j] = R[4 * i + 3];
-                }
-#pragma unroll
-                for (int j = 0; j < kSweepBatch; ++j) {
-                    const int i = i0 + j;
-                    if (i < hi) {
-                        const double a = (i == lo) ? 0.0 : rl[j];          // cut below the halo
-                        const double c = (i == hi - 1) ? 0.0 : ru[j];      // cut above the halo
-                        const double den = rd[j] - a * pc;
-                        pc = c / den;
-                        pd = (rb[j] - a * pd) / den;
-                        cw[2 * (i - lo)] = pc;
-                        cw[2 * (i - lo) + 1] = pd;
-                    }
-                }
-            }
-            double x = 0.0;
-            for (int i1 = hi - 1; i1 >= lo; i1 -= kSweepBatch) {
-                double wc[kSweepBatch], wd[kSweepBatch];
-#pragma unroll
-                for (int j = 0; j < kSweepBatch; ++j) {
-                    const int i = (i1 - j >= lo) ? i1 - j : lo;
-                    wc[j] = cw[2 * (i - lo)];
-                    wd[j] = cw[2 * (i - lo) + 1];
-                }
-#pragma unroll
-                for (int j = 0; j < kSweepBatch; ++j) {
-                    const int i = i1 - j;
-                    if (i >= lo) {
-                        x = wd[j] - wc[j] * x;
-                        if (i >= s0 && i < e0) S[i] = x;
-                    }
-                }
-            }
+            for (int k = 0; k < kSeg + 4; ++k) { wt[k] = nt[k]; wy[k] = ny[k]; }
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < len - 1; i += blockDim.x) {
-        const double dt = T[5 * (i + 1)] - T[5 * i];
-        const double yi = Y[3 * i], yp = Y[3 * (i + 1)];
-        const double slope = (yp - yi) / dt;
-        const double si = S[i], sp = S[i + 1];
-        const double tt = (si + sp - 2 * slope) / dt;
-        C[12 * i] = yi;
-        C[12 * i + 1] = si;
-        C[12 * i + 2] = (slope - si) / dt - tt;
-        C[12 * i + 3] = tt / dt;
-    }
-    if (threadIdx.x == 0) {
-        C[12 * (len - 1)] = Y[3 * (len - 1)];
-        C[12 * (len - 1) + 1] = C[12 * (len - 1) + 2] = C[12 * (len - 1) + 3] = 0.0;
+    // back substitution, segment by segment from the top; xn = slope of the row above
+    double xn = 0.0;
+    int sg = nseg - 1;
+    seg_load(T, Y, nser, lo + sg * kSeg, len, wt, wy);
+    double cpc = ck[2 * sg], cpd = ck[2 * sg + 1];
+    for (; sg >= 0; --sg) {
+        const int start = lo + sg * kSeg, end = (start + kSeg < hi) ? start + kSeg : hi;
+        if (end <= s0) break;   // the rows below this lane's chunk need no slopes
+        double pcs[kSeg], pds[kSeg];
+        pc = cpc;
+        pd = cpd;
+        const bool more = sg > 0 && start > s0;
+        if (more) {
+            seg_load(T, Y, nser, start - kSeg, len, nt, ny);
+            cpc = ck[2 * (sg - 1)];
+            cpd = ck[2 * (sg - 1) + 1];
+        }
+        seg_forward<true>(wt, wy, start, end, lo, hi, len, rfirst, rlast, pc, pd, pcs, pds);
+#pragma unroll
+        for (int j = kSeg - 1; j >= 0; --j) {
+            const int i = start + j;
+            if (i < end) {
+                const double xi = pds[j] - pcs[j] * xn;   // the cut row hi-1 has c' = 0
+                if (i >= s0 && i < e0) {
+                    double* c = C + (size_t)cs * i;
+                    if (i < len - 1) {
+                        const double yi = wy[j + 2], yp = wy[j + 3];
+                        const double dt = wt[j + 3] - wt[j + 2];
+                        const double slope = (yp - yi) / dt;
+                        const double tt = (xi + xn - 2 * slope) / dt;
+                        c[0] = yi;
+                        c[1] = xi;
+                        c[2] = (slope - xi) / dt - tt;
+                        c[3] = tt / dt;
+                    } else {
+                        c[0] = wy[j + 2];
+                        c[1] = c[2] = c[3] = 0.0;
+                    }
+                }
+                xn = xi;
+            }
+        }
+        if (!more) break;
+#pragma unroll
+        for (int k = 0; k < kSeg + 4; ++k) { wt[k] = nt[k]; wy[k] = ny[k]; }
     }
 }
 
 // Uniform resample of h+ = amp * Re[(Re S + i Im S)(t) e^{-2 i phi(t)}] at
 // t_j = t_0 + j dT_M (dT_M = dt / (M MTSUN)), j < N_b = floor((t_last - t_0)/dT_M) + 1,
 // zero for N_b <= j < N.  Interval: searchsorted(t, t_j, 'right') - 1 over the knots
-// (eob/seobnrv4phm.py:241-272).  grid (blocks over j, binaries).
+// (eob/seobnrv4phm.py:241-272).  grid (blocks, binaries); a block takes spans of
+// kResSpan consecutive samples, stages the knot times of their intervals in LDS
+// (knot_window) and each thread searches there.
+constexpr int kResPerThread = 16;
+constexpr int kResSpan = kBlock * kResPerThread;
 __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restrict__ dense, const double* __restrict__ coef,
                                                            const int* __restrict__ num_steps, const double* __restrict__ amp,
                                                            const double* __restrict__ dT_M, int n, int max_step, int N,
@@ -266,32 +326,176 @@ __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restr
     if (nb > N) nb = N;
     if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = (int)nb;
     const double a = amp[b];
-    for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) {
-        double val = 0.0, valc = 0.0;
-        if (j < nb) {
-            const double tj = t0 + (double)j * dT;
-            int lo = 0, hi = len;
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (T[5 * mid] <= tj) lo = mid + 1;
-                else hi = mid;
-            }
-            int ind = lo - 1;
-            if (ind < 0) ind = 0;
-            const double x = tj - T[5 * ind];
-            const double x2 = x * x, x3 = x2 * x;
-            const double* c = C + 12 * ind;
-            const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
-            const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
-            const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
-            double sp, cp;
-            sincos(2.0 * ph, &sp, &cp);
-            // h+ - i hx = (sr + i si)(cos 2ph - i sin 2ph) (eob/seobnrv4phm.py:720, 742-744)
-            val = a * (sr * cp + si * sp);
-            valc = a * (sr * sp - si * cp);
+    __shared__ KnotWindow<kBlock> win;
+    const long long nspan = ((long long)N + kResSpan - 1) / kResSpan;
+    for (long long sp0 = blockIdx.x; sp0 < nspan; sp0 += gridDim.x) {   // the same trip count on every thread
+        const long long j0 = sp0 * kResSpan;
+        if (j0 < nb) {
+            const long long jmax = (j0 + kResSpan < nb ? j0 + kResSpan : nb) - 1;
+            knot_window<kBlock>(win, T, 5, len, t0 + (double)j0 * dT, t0 + (double)jmax * dT);
         }
-        out[j] = val;
-        if (outc) outc[j] = valc;
+#pragma unroll 4
+        for (int u = 0; u < kResPerThread; ++u) {
+            const long long j = j0 + (long long)u * kBlock + threadIdx.x;
+            if (j >= N) break;
+            double val = 0.0, valc = 0.0;
+            if (j < nb) {
+                const double tj = t0 + (double)j * dT;
+                const int ind = knot_interval<kBlock>(win, T, 5, len, tj);
+                const double x = tj - T[5 * ind];
+                const double x2 = x * x, x3 = x2 * x;
+                const double* c = C + 12 * ind;
+                const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
+                const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
+                const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
+                double sp, cp;
+                sincos(2.0 * ph, &sp, &cp);
+                // the template (sr + i si)(cos 2ph - i sin 2ph); h+ and hx are its real and
+                // imaginary parts, the series the reference transforms (eob/seobnrv4phm.py:720, 742-744)
+                val = a * (sr * cp + si * sp);
+                valc = a * (si * cp - sr * sp);
+            }
+            out[j] = val;
+            if (outc) outc[j] = valc;
+        }
+        __syncthreads();   // the window is rewritten by the next span
+    }
+}
+
+// Uniform resample of the mode sum h = amp sum_j S_j(t) e^{-i m_j phi(t)} (the
+// reference's modes *= exp(-1j m phi_interp) and template = modes.sum,
+// eob/seobnrv4phm.py:2366-2372, 720) at t_j = t_0 + j dT_M, j < N_b, zero past;
+// hplus, hcross = Re h, Im h (the two series it transforms, :742-744).  coef
+// [n][max_step][4 (2 nm + 1)]: series j < nm Re S_j, nm + j Im S_j, 2 nm phi.
+// e^{-i m phi} is formed as the m-th power of e^{-i phi}.
+constexpr int kMaxModeM = 8;
+struct ModeMs {
+    int nm;
+    int m[15];
+};
+__global__ __launch_bounds__(kBlock) void k_resample_modes(const double* __restrict__ dense, const double* __restrict__ coef,
+                                                           const int* __restrict__ num_steps, const double* __restrict__ amp,
+                                                           const double* __restrict__ dT_M, int max_step, int N, ModeMs mm,
+                                                           double* __restrict__ hplus, double* __restrict__ hcross,
+                                                           int* __restrict__ nsamp) {
+    const int b = blockIdx.y;
+    const int len = num_steps[b];
+    const int nm = mm.nm, S = 2 * nm + 1, cs = 4 * S;
+    double* out = hplus + (size_t)b * N;
+    double* outc = hcross + (size_t)b * N;
+    if (len < 2) {
+        for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (long long)gridDim.x * blockDim.x) {
+            out[j] = 0.0;
+            outc[j] = 0.0;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = 0;
+        return;
+    }
+    const double* T = dense + (size_t)b * max_step * 5;
+    const double* C = coef + (size_t)b * max_step * cs;
+    const double t0 = T[0], tl = T[5 * (len - 1)];
+    const double dT = dT_M[b];
+    long long nb = (long long)((tl - t0) / dT) + 1;
+    if (nb > N) nb = N;
+    if (blockIdx.x == 0 && threadIdx.x == 0) nsamp[b] = (int)nb;
+    const double a = amp[b];
+    __shared__ KnotWindow<kBlock> win;
+    const long long nspan = ((long long)N + kResSpan - 1) / kResSpan;
+    for (long long sp0 = blockIdx.x; sp0 < nspan; sp0 += gridDim.x) {
+        const long long j0 = sp0 * kResSpan;
+        if (j0 < nb) {
+            const long long jmax = (j0 + kResSpan < nb ? j0 + kResSpan : nb) - 1;
+            knot_window<kBlock>(win, T, 5, len, t0 + (double)j0 * dT, t0 + (double)jmax * dT);
+        }
+        for (int u = 0; u < kResPerThread; ++u) {
+            const long long j = j0 + (long long)u * kBlock + threadIdx.x;
+            if (j >= N) break;
+            double hr = 0.0, hi = 0.0;
+            if (j < nb) {
+                const double tj = t0 + (double)j * dT;
+                const int ind = knot_interval<kBlock>(win, T, 5, len, tj);
+                const double x = tj - T[5 * ind];
+                const double x2 = x * x, x3 = x2 * x;
+                const double* c = C + (size_t)cs * ind;
+                const double* cp = c + 4 * (2 * nm);
+                const double ph = cp[0] + cp[1] * x + cp[2] * x2 + cp[3] * x3;
+                double sp, cph;
+                sincos(ph, &sp, &cph);
+                // e^{-i k phi}, k = 1 .. 8
+                double er[kMaxModeM + 1], ei[kMaxModeM + 1];
+                er[0] = 1.0; ei[0] = 0.0;
+#pragma unroll
+                for (int k = 1; k <= kMaxModeM; ++k) {
+                    er[k] = er[k - 1] * cph + ei[k - 1] * sp;
+                    ei[k] = ei[k - 1] * cph - er[k - 1] * sp;
+                }
+                for (int q = 0; q < nm; ++q) {
+                    const double* cr = c + 4 * q;
+                    const double* ci = c + 4 * (nm + q);
+                    const double sr = cr[0] + cr[1] * x + cr[2] * x2 + cr[3] * x3;
+                    const double si = ci[0] + ci[1] * x + ci[2] * x2 + ci[3] * x3;
+                    const int k = mm.m[q];
+                    double pr_ = 1.0, pi_ = 0.0;
+#pragma unroll
+                    for (int kk = 1; kk <= kMaxModeM; ++kk)
+                        if (kk == k) { pr_ = er[kk]; pi_ = ei[kk]; }
+                    hr += sr * pr_ - si * pi_;
+                    hi += sr * pi_ + si * pr_;
+                }
+                hr *= a;
+                hi *= a;
+            }
+            out[j] = hr;
+            outc[j] = hi;
+        }
+        __syncthreads();   // the window is rewritten by the next span
+    }
+}
+
+// TD2 (src/interpolate.cu:441-549), TDInterp_wrap2's kernel: per binary b and
+// sample i < lengths[b], sum over modes of (re + i im) e^{-i m phi} with every
+// series evaluated at the caller's interval inds[b][i] (t - ts[b][ind], data
+// time dataTime[i] shared by the binaries).  Series s of binary b starts at
+// b (2M + 1) old_length + s * stride: stride old_lengths[b] as the reference
+// indexes (correct only when every binary fills old_length), or old_length for
+// the padded layout (legacy_stride 0).  Intervals are clamped into
+// [0, old_length - 1].  grid (blocks over samples, binaries).
+__global__ __launch_bounds__(kBlock) void k_td_interp2(double2* __restrict__ out, const double* __restrict__ dataTime,
+                                                       const double* __restrict__ ts, const double* __restrict__ y,
+                                                       const double* __restrict__ c1, const double* __restrict__ c2,
+                                                       const double* __restrict__ c3, int old_length, const int* __restrict__ old_lengths,
+                                                       int data_length, int numModes, const int* __restrict__ ms,
+                                                       const int* __restrict__ inds, const int* __restrict__ lengths, int max_length,
+                                                       int legacy_stride) {
+    const int b = blockIdx.y;
+    // sample counts and strides are clamped to the arrays' extents (the reference
+    // trusts them; for valid inputs the clamps change nothing)
+    int n = lengths[b];
+    n = n < max_length ? n : max_length;
+    n = n < data_length ? n : data_length;
+    int olh = old_lengths[b];
+    olh = olh < 0 ? 0 : (olh > old_length ? old_length : olh);
+    const long long stride = legacy_stride ? olh : old_length;
+    const size_t base = (size_t)b * (2 * numModes + 1) * old_length;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int ind = inds[(size_t)max_length * b + i];
+        ind = ind < 0 ? 0 : (ind > old_length - 1 ? old_length - 1 : ind);
+        const double x = dataTime[i] - ts[(size_t)b * old_length + ind];
+        const double x2 = x * x, x3 = x * x2;
+        size_t k = base + (size_t)(2 * numModes) * stride + ind;
+        const double phi = y[k] + c1[k] * x + c2[k] * x2 + c3[k] * x3;
+        double hr = 0.0, hi = 0.0;
+        for (int q = 0; q < numModes; ++q) {
+            k = base + (size_t)q * stride + ind;
+            const double re = y[k] + c1[k] * x + c2[k] * x2 + c3[k] * x3;
+            k = base + (size_t)(numModes + q) * stride + ind;
+            const double im = y[k] + c1[k] * x + c2[k] * x2 + c3[k] * x3;
+            double s, c;
+            sincos(-((double)ms[q] * phi), &s, &c);
+            hr += re * c - im * s;
+            hi += re * s + im * c;
+        }
+        out[(size_t)b * data_length + i] = make_double2(hr, hi);
     }
 }
 
@@ -475,15 +679,25 @@ int eobhip_all_in_one_likelihood(double* temp_sum, const double* hp_fd, const do
 
 size_t eobhip_spline_scratch_doubles(int n, int max_step) {
     if (n < 0 || max_step < 1) return 0;
-    return (size_t)n * 3 * (size_t)spline_scratch_per_interp(max_step);
+    return (size_t)n * (size_t)spline_scratch_per_bin(max_step, 3);
 }
 
 int eobhip_spline_build_batched(const double* dense, const double* knots, const int* num_steps, int n, int max_step, double* coef,
                                 double* scratch, void* stream) {
+    return eobhip_spline_build_series(dense, knots, num_steps, n, max_step, 3, coef, scratch, stream);
+}
+
+size_t eobhip_spline_series_scratch_doubles(int n, int max_step, int nser) {
+    if (n < 0 || max_step < 1 || nser < 1 || nser > kMaxSeries) return 0;
+    return (size_t)n * (size_t)spline_scratch_per_bin(max_step, nser);
+}
+
+int eobhip_spline_build_series(const double* dense, const double* knots, const int* num_steps, int n, int max_step, int nser,
+                               double* coef, double* scratch, void* stream) {
     EOBHIP_REQUIRE(n >= 0 && max_step >= 1, "bad sizes");
-    EOBHIP_REQUIRE(3LL * n <= 0x7fffffffLL, "too many interpolants");
+    EOBHIP_REQUIRE(nser >= 1 && nser <= kMaxSeries, "nser must be 1 .. 32");
     if (n == 0) return EOBHIP_OK;
-    hipLaunchKernelGGL(k_spline_build_par, dim3(3 * n), dim3(kSplineThreads), 0, as_stream(stream), dense, knots, num_steps, n, max_step,
+    hipLaunchKernelGGL(k_spline_build_seg, dim3(n), dim3(kSplineLanes), 0, as_stream(stream), dense, knots, num_steps, n, max_step, nser,
                        coef, scratch);
     return check_launch("spline_build");
 }
@@ -492,11 +706,46 @@ int eobhip_resample_hplus(const double* dense, const double* coef, const int* nu
                           int max_step, int N, double* hplus, int* nsamp, void* stream) {
     EOBHIP_REQUIRE(n >= 0 && N >= 1 && n <= 65535, "bad sizes");
     if (n == 0) return EOBHIP_OK;
-    int bx = (int)((N + (long long)kBlock * 4 - 1) / ((long long)kBlock * 4));
+    int bx = (int)((N + (long long)kResSpan - 1) / kResSpan);
     if (bx > 4096) bx = 4096;
     hipLaunchKernelGGL(k_resample_hplus, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, n, max_step, N,
                        hplus, nsamp, nullptr);
     return check_launch("resample_hplus");
+}
+
+int eobhip_resample_modes(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
+                          int max_step, int N, int nm, const int* ms, double* hplus, double* hcross, int* nsamp, void* stream) {
+    EOBHIP_REQUIRE(n >= 0 && N >= 1 && n <= 65535, "bad sizes");
+    EOBHIP_REQUIRE(nm >= 1 && nm <= 15, "nm must be 1 .. 15");
+    EOBHIP_REQUIRE(hplus && hcross, "hplus and hcross are required");
+    ModeMs mm;
+    mm.nm = nm;
+    for (int q = 0; q < nm; ++q) {
+        EOBHIP_REQUIRE(ms[q] >= 1 && ms[q] <= kMaxModeM, "m must be 1 .. 8");
+        mm.m[q] = ms[q];
+    }
+    if (n == 0) return EOBHIP_OK;
+    int bx = (int)((N + (long long)kResSpan - 1) / kResSpan);
+    if (bx > 4096) bx = 4096;
+    hipLaunchKernelGGL(k_resample_modes, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, max_step, N,
+                       mm, hplus, hcross, nsamp);
+    return check_launch("resample_modes");
+}
+
+int eobhip_TDInterp2(void* templateChannels, const double* dataTime, const double* ts, const double* propArrays, const double* c1,
+                     const double* c2, const double* c3, int old_length, const int* old_lengths, int data_length, int numBinAll,
+                     int numModes, const int* ls, const int* ms, const int* inds, const int* lengths, int max_length, int legacy_stride,
+                     void* stream) {
+    (void)ls;   // the reference reads ls into shared memory but combines with m only
+    EOBHIP_REQUIRE(numBinAll >= 0 && numModes >= 1 && old_length >= 1 && max_length >= 0 && data_length >= 0, "bad sizes");
+    EOBHIP_REQUIRE(numBinAll <= 65535, "numBinAll must be <= 65535 (grid y dimension)");
+    if (numBinAll == 0 || max_length == 0) return EOBHIP_OK;
+    int bx = (max_length + kBlock - 1) / kBlock;
+    if (bx > 4096) bx = 4096;
+    hipLaunchKernelGGL(k_td_interp2, dim3(bx, numBinAll), dim3(kBlock), 0, as_stream(stream), static_cast<double2*>(templateChannels),
+                       dataTime, ts, propArrays, c1, c2, c3, old_length, old_lengths, data_length, numModes, ms, inds, lengths, max_length,
+                       legacy_stride);
+    return check_launch("TDInterp2");
 }
 
 int eobhip_resample_hpc(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
@@ -504,7 +753,7 @@ int eobhip_resample_hpc(const double* dense, const double* coef, const int* num_
     EOBHIP_REQUIRE(n >= 0 && N >= 1 && n <= 65535, "bad sizes");
     EOBHIP_REQUIRE(hcross != nullptr, "hcross is required");
     if (n == 0) return EOBHIP_OK;
-    int bx = (int)((N + (long long)kBlock * 4 - 1) / ((long long)kBlock * 4));
+    int bx = (int)((N + (long long)kResSpan - 1) / kResSpan);
     if (bx > 4096) bx = 4096;
     hipLaunchKernelGGL(k_resample_hplus, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, n, max_step, N,
                        hplus, nsamp, hcross);

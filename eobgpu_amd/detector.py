@@ -58,7 +58,9 @@ def detector_signal(hp_fd, hc_fd, Fplus, Fcross, time_shift, df, k_min=0, out=No
 
 
 def resample_hpc(batch, total_mass_msun, dt, N, amp=None):
-    """h+ and hx [n][N] of a batch whose splines are built (h+ - i hx = amp h22)."""
+    """h+ and hx [n][N] of a batch whose splines are built: the real and imaginary
+    parts of amp h22(t_j), the two series the reference transforms
+    (template.real, template.imag, eob/seobnrv4phm.py:742-744)."""
     from .pipeline import MTSUN_SI
     dev = batch.device
     n = batch.n

@@ -107,6 +107,10 @@ def make_additional_args(m1, m2, chi1, chi2, omega0=None, K=0.0, d5=0.0, dSO=0.0
     return add
 
 
+# the modes the reference's SEOBNRv4HM evaluates (eob/eobstuff.py:56)
+REFERENCE_MODES = ((2, 2), (2, 1), (3, 3), (4, 4), (5, 5))
+
+
 class EOBBatch:
     """Device-resident tables and trajectory buffers for n binaries."""
 
@@ -297,6 +301,75 @@ class EOBBatch:
              dT_M.data_ptr(), self.n, self.max_step, int(N), out.data_ptr(), self.nsamp.data_ptr(), current_stream())
         self._keep = (dT_M, amp)
         return out
+
+
+    # ------------------------------------------------------------- higher modes
+    def hlm_modes_knots(self, modes=REFERENCE_MODES, out=None):
+        """The modes (l, m) on every binary's knots (get_hlms, eob/seobnrv4phm.py:2138-2221,
+        and the series compute_full_waveform splines, :2336-2356): knots
+        [n][max_step][2 M + 1] = Re(h_j e^{i m_j phi}) for the M modes, their Im, phi."""
+        ls, ms = _mode_arrays(modes)
+        nm = len(ls)
+        L = lib()
+        key = (tuple(ls), tuple(ms))
+        if getattr(self, "_wave_key", None) != key:
+            self.wtab = torch.empty(int(L.eobhip_wave_mode_bytes()) * self.n * nm, dtype=torch.uint8, device=self.device)
+            self._wave_key = key
+        call("eobhip_wave_tables", self.add.data_ptr(), self.n, self.use_hm, self.bindata.data_ptr(), self.modes.data_ptr(), nm,
+             _host_ints(ls), _host_ints(ms), self.wtab.data_ptr(), current_stream())
+        if out is None:
+            out = torch.zeros((self.n, self.max_step, 2 * nm + 1), dtype=torch.float64, device=self.device)
+        call("eobhip_hlm_modes_knots", self.dense.data_ptr(), self.num_steps.data_ptr(), self.bindata.data_ptr(), self.wtab.data_ptr(),
+             self.add.data_ptr(), self.n, self.max_step, nm, out.data_ptr(), current_stream())
+        self.mode_knots, self.mode_ms = out, ms
+        return out
+
+    def build_mode_splines(self):
+        """Not-a-knot splines of the 2 M + 1 series of ``hlm_modes_knots``:
+        coef [n][max_step][4 (2 M + 1)]."""
+        nser = self.mode_knots.shape[2]
+        L = lib()
+        self.mode_coef = torch.empty((self.n, self.max_step, 4 * nser), dtype=torch.float64, device=self.device)
+        scratch = torch.empty(int(L.eobhip_spline_series_scratch_doubles(self.n, self.max_step, nser)), dtype=torch.float64,
+                              device=self.device)
+        call("eobhip_spline_build_series", self.dense.data_ptr(), self.mode_knots.data_ptr(), self.num_steps.data_ptr(), self.n,
+             self.max_step, nser, self.mode_coef.data_ptr(), scratch.data_ptr(), current_stream())
+        self._keep_scratch = scratch
+        return self.mode_coef
+
+    def resample_modes(self, total_mass_msun, dt, N, amp=None):
+        """(h+, hx) [n][N]: the real and imaginary parts of amp * sum_j h_j(t_j) on
+        t_j = t_0 + j dt (seconds), j < N, zero past the last knot -- the reference's
+        template over its modes (eob/seobnrv4phm.py:2366-2372, 720, 742-744)."""
+        M = torch.as_tensor(np.broadcast_to(np.asarray(total_mass_msun, dtype=np.float64), (self.n,)).copy(), device=self.device)
+        dT_M = (dt / (M * MTSUN_SI)).contiguous()
+        a = torch.as_tensor(np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (self.n,)).copy(),
+                            device=self.device)
+        hp = torch.empty((self.n, N), dtype=torch.float64, device=self.device)
+        hc = torch.empty_like(hp)
+        self.nsamp = torch.zeros(self.n, dtype=torch.int32, device=self.device)
+        ms = self.mode_ms
+        call("eobhip_resample_modes", self.dense.data_ptr(), self.mode_coef.data_ptr(), self.num_steps.data_ptr(), a.data_ptr(),
+             dT_M.data_ptr(), self.n, self.max_step, int(N), len(ms), _host_ints(ms), hp.data_ptr(), hc.data_ptr(),
+             self.nsamp.data_ptr(), current_stream())
+        self._keep = (dT_M, a)
+        return hp, hc
+
+
+def _mode_arrays(modes):
+    modes = [(int(l), int(m)) for l, m in modes]
+    if not 1 <= len(modes) <= 15:
+        raise ValueError("between 1 and 15 modes")
+    for l, m in modes:
+        if not (2 <= l <= 8 and 1 <= m <= l):
+            raise ValueError(f"mode {(l, m)} outside 2 <= l <= 8, 1 <= m <= l")
+    return [l for l, _ in modes], [m for _, m in modes]
+
+
+def _host_ints(xs):
+    """A host int32 array for a by-value pointer argument (kept alive by ctypes' call)."""
+    import ctypes
+    return (ctypes.c_int * len(xs))(*xs)
 
 
 def rfft_batched(h, out=None):

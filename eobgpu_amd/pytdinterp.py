@@ -30,10 +30,21 @@ def all_in_one_likelihood(temp_sum, hp_fd, hc_fd, data, psd, Fplus, Fcross, time
     sync()
 
 
-def TDInterp_wrap2(*args, **kwargs):
-    """src/tdinterp.pyx:55-68 (TD2 mode combine).  Dead on the reference's main path
-    (eob/seobnrv4phm.py evaluates splines with cupy); use EOBBatch.resample_hplus."""
-    raise NotImplementedError("TDInterp_wrap2 is not provided; see eobgpu_amd.pipeline.EOBBatch.resample_hplus")
+@pointer_adjust
+def TDInterp_wrap2(templateChannels, dataTime, ts, propArrays, c1, c2, c3, old_length, old_lengths, data_length, numBinAll, numModes,
+                   ls, ms, inds, lengths, max_length, legacy_stride=True):
+    """src/tdinterp.pyx:60-75 -> TDInterp2 (src/interpolate.cu:441-549): the mode sum
+    sum_j (re_j + i im_j) e^{-i m_j phi} at dataTime[i], i < lengths[b], into
+    templateChannels [numBinAll][data_length] (complex128).
+
+    legacy_stride=True indexes binary b's series with stride old_lengths[b], as the
+    reference does (src/interpolate.cu:521-527), which is right only when every
+    binary has old_length knots; False uses the padded stride old_length (the
+    layout [numBinAll][2 numModes + 1][old_length] the arrays have).  ls, ms, inds,
+    lengths, old_lengths are int32 device arrays."""
+    call("eobhip_TDInterp2", templateChannels, dataTime, ts, propArrays, c1, c2, c3, int(old_length), old_lengths, int(data_length),
+         int(numBinAll), int(numModes), ls, ms, inds, lengths, int(max_length), 1 if legacy_stride else 0, current_stream())
+    sync()
 
 
 def TDInterp_wrap(*args, **kwargs):

@@ -237,18 +237,55 @@ int eobhip_hlm22_knots(const double* dense, const int* num_steps, const void* bi
 
 /* Not-a-knot splines of the three knot series (phi and Re, Im of h22 e^{2 i phi}) per
  * binary (CubicSplineInterpolantTD / interpolate_TD semantics, src/interpolate.cu:11-308),
- * one workgroup per series.  coef [n][max_step][12] = (y, c1, c2, c3) x 3;
+ * one wavefront per binary.  coef [n][max_step][12] = (y, c1, c2, c3) x 3;
  * scratch: eobhip_spline_scratch_doubles(n, max_step) doubles (binary b's part starts at
  * b * eobhip_spline_scratch_doubles(1, max_step)). */
 size_t eobhip_spline_scratch_doubles(int n, int max_step);
+/* The same splines for any number of series: knots [n][max_step][nser],
+ * coef [n][max_step][4 nser], 1 <= nser <= 32; scratch:
+ * eobhip_spline_series_scratch_doubles(n, max_step, nser) doubles (binary b's part
+ * starts at b * eobhip_spline_series_scratch_doubles(1, max_step, nser)). */
+size_t eobhip_spline_series_scratch_doubles(int n, int max_step, int nser);
+int eobhip_spline_build_series(const double* dense, const double* knots, const int* num_steps, int n, int max_step, int nser,
+                               double* coef, double* scratch, void* stream);
 int eobhip_spline_build_batched(const double* dense, const double* knots, const int* num_steps, int n, int max_step,
                                 double* coef, double* scratch, void* stream);
 /* h+ = amp * Re[h22(t_j)] on t_j = t_0 + j dT_M (dT_M [n] in units of M), j < N; zero past the
  * last knot; nsamp[n] = samples covered by the knots (eob/seobnrv4phm.py:2319-2372). */
 int eobhip_resample_hplus(const double* dense, const double* coef, const int* num_steps, const double* amp,
                           const double* dT_M, int n, int max_step, int N, double* hplus, int* nsamp, void* stream);
-/* eobhip_resample_hplus with the cross polarization as well: h+ - i hx = amp * h22(t_j)
- * (the reference's template_channels, eob/seobnrv4phm.py:720, 742-744); hcross [n][N]. */
+/* eobhip_resample_hplus with the cross polarization as well: h+ + i hx = amp * h22(t_j),
+ * the real and imaginary parts of the reference's template_channels, which it
+ * transforms as the two polarizations (eob/seobnrv4phm.py:720, 742-744); hcross [n][N]. */
+/* Higher modes of a batch (get_hlms for ells, mms, eob/seobnrv4phm.py:2138-2221, through
+ * EOBGetSpinFactorizedWaveform, src/EOB.cu:1236-1613): the waveform tables of nm <= 15
+ * modes per binary (wtab: n * nm * eobhip_wave_mode_bytes() bytes) from the batch's
+ * additional args and tables (use_hm as at setup), then the modes on the knots,
+ * knots [n][max_step][2 nm + 1] = Re(h_j e^{i m_j phi}), Im(...), phi -- the series
+ * compute_full_waveform splines (:2336-2356). */
+size_t eobhip_wave_mode_bytes(void);
+int eobhip_wave_tables(const double* add, int n, int use_hm, const void* bindata, const void* modes, int nm, const int* ls,
+                       const int* ms, void* wtab, void* stream);
+int eobhip_hlm_modes_knots(const double* dense, const int* num_steps, const void* bindata, const void* wtab, const double* add, int n,
+                           int max_step, int nm, double* knots, void* stream);
+/* The mode sum on the uniform grid from splines of those 2 nm + 1 series (coef
+ * [n][max_step][4 (2 nm + 1)], eobhip_spline_build_series): h = amp sum_j S_j(t_j)
+ * e^{-i m_j phi(t_j)}, hplus = Re h, hcross = Im h [n][N] (the template's real and
+ * imaginary parts, eob/seobnrv4phm.py:2366-2372, 720, 742-744); ms: host array, 1 <= m <= 8. */
+int eobhip_resample_modes(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
+                          int max_step, int N, int nm, const int* ms, double* hplus, double* hcross, int* nsamp, void* stream);
+/* TDInterp_wrap2 / TDInterp2 (src/tdinterp.pyx:60-75, src/interpolate.cu:441-549):
+ * templateChannels complex [numBinAll][data_length] = sum over modes of (re + i im)
+ * e^{-i m phi} at dataTime[i], i < lengths[b], intervals inds [numBinAll][max_length];
+ * propArrays, c1, c2, c3 [numBinAll][2 numModes + 1][old_length] (Re modes, Im modes,
+ * phi); ts [numBinAll][old_length]; ls, ms, inds, lengths, old_lengths device int32.
+ * legacy_stride 1 indexes the series of binary b with stride old_lengths[b] as the
+ * reference does (src/interpolate.cu:521-527; right only when every binary has
+ * old_length knots); 0 uses the padded stride old_length. */
+int eobhip_TDInterp2(void* templateChannels, const double* dataTime, const double* ts, const double* propArrays, const double* c1,
+                     const double* c2, const double* c3, int old_length, const int* old_lengths, int data_length, int numBinAll,
+                     int numModes, const int* ls, const int* ms, const int* inds, const int* lengths, int max_length, int legacy_stride,
+                     void* stream);
 int eobhip_resample_hpc(const double* dense, const double* coef, const int* num_steps, const double* amp, const double* dT_M, int n,
                         int max_step, int N, double* hplus, double* hcross, int* nsamp, void* stream);
 

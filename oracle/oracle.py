@@ -265,7 +265,7 @@ def hlm22(r, phi, v, Hreal, eta, vPhiIn, prefix, coeffs, h22_calib=0.0):
 
 
 def hlm(l, m, r, phi, pphi, v, Hreal, eta, vPhiIn, prefix, coeffs, h22_calib=0.0):
-    """EOBGetSpinFactorizedWaveform for (l, m) in {(2,2), (2,1), (3,3), (4,4), (5,5)} on knots
+    """EOBGetSpinFactorizedWaveform for any (l, m), 2 <= l <= 8, 1 <= m <= l, on knots
     [numSys][L] (src/EOB.cu:1236-1613); prefix complex [numSys]; returns complex [numSys][L]."""
     r = _c(r); phi = _c(phi); pphi = _c(pphi); v = _c(v); Hreal = _c(Hreal); vPhiIn = _c(vPhiIn); eta = _c(eta); coeffs = _c(coeffs)
     pf = _c(np.asarray(prefix, dtype=np.complex128).view(np.float64))
@@ -338,6 +338,59 @@ def get_hlm22(traj, lengths, add, coeffs, h22_calib=0.0):
     out = np.zeros((nbin, L, 2))
     lib().orc_get_hlm22(_p(out), _p(traj), _pi(lengths), nbin, L, _p(add), _p(coeffs), ctypes.c_double(h22_calib))
     return out[..., 0] + 1j * out[..., 1]
+
+
+def flux_mode_abs2(l, m, v, Hreal, pphi, vPhi2, prefix_abs, eta, coeff_row, h22_calib=0.0):
+    """|h_lm|^2 as the flux evaluates it (src/EOB.cu:1706-1941), one binary's record."""
+    v = _c(v); Hreal = _c(Hreal); pphi = _c(pphi); vPhi2 = _c(vPhi2); row = _c(coeff_row)
+    out = np.zeros(v.size)
+    lib().orc_flux_mode_abs2(_p(out), int(l), int(m), _p(v), _p(Hreal), _p(pphi), _p(vPhi2), int(v.size), ctypes.c_double(prefix_abs),
+                             ctypes.c_double(eta), ctypes.c_double(h22_calib), _p(row))
+    return out
+
+
+def get_hlm(l, m, traj, lengths, add, coeffs, h22_calib=0.0):
+    """Mode (l, m) on knots for traj [nbin][4][len] (eob/seobnrv4phm.py:2138-2221)."""
+    traj = _c(traj); lengths = _c(lengths, np.int32); add = _c(add); coeffs = _c(coeffs)
+    nbin, _, L = traj.shape
+    out = np.zeros((nbin, L, 2))
+    lib().orc_get_hlm(_p(out), int(l), int(m), _p(traj), _pi(lengths), nbin, L, _p(add), _p(coeffs), ctypes.c_double(h22_calib))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def td_interp2(dataTime, ts, propArrays, c1, c2, c3, old_length, old_lengths, data_length, ms, inds, lengths, max_length,
+               legacy_stride=True):
+    """TD2 (src/interpolate.cu:441-549) restated with numpy: per binary b and sample
+    i < lengths[b], sum over modes of (re + i im) e^{-i m phi}, each series the
+    cubic at the sample's interval inds[b][i]; series s of binary b starts at
+    b (2M+1) old_length + s * stride with stride = old_lengths[b] (the reference,
+    legacy_stride) or old_length (the padded layout).  Returns complex
+    [numBinAll][data_length] (zero past lengths[b])."""
+    nbin = len(old_lengths)
+    M = len(ms)
+    out = np.zeros((nbin, data_length), dtype=np.complex128)
+    y = np.asarray(propArrays, dtype=np.float64); c1 = np.asarray(c1); c2 = np.asarray(c2); c3 = np.asarray(c3)
+    ts = np.asarray(ts, dtype=np.float64).reshape(nbin, old_length)
+    inds = np.asarray(inds).reshape(nbin, max_length)
+    for b in range(nbin):
+        n = int(lengths[b])
+        if n == 0:
+            continue
+        t = np.asarray(dataTime[:n], dtype=np.float64)
+        ind = inds[b, :n].astype(np.int64)
+        x = t - ts[b, ind]
+        stride = int(old_lengths[b]) if legacy_stride else old_length
+        base = b * (2 * M + 1) * old_length
+
+        def ev(sub):
+            k = base + sub * stride + ind
+            return y[k] + c1[k] * x + c2[k] * x * x + c3[k] * x * x * x
+        phi = ev(2 * M)
+        acc = np.zeros(n, dtype=np.complex128)
+        for j, m in enumerate(ms):
+            acc += (ev(j) + 1j * ev(M + j)) * np.exp(-1j * (m * phi))
+        out[b, :n] = acc
+    return out
 
 
 # ------------------------------------------------------------------- splines

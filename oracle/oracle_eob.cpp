@@ -635,13 +635,13 @@ static cplx hlm22(double r, double phi, double v, double Hreal, const FacCoeffs&
     return h * hN;
 }
 
-// EOBGetSpinFactorizedWaveform (src/EOB.cu:1236-1613) for the modes the
-// higher-mode waveform uses, (2,2), (2,1), (3,3), (4,4), (5,5):
-// h_lm = T_lm e^{i delta_lm} S_lm (rho_lm^l + f_lm) h^N_lm, with rho^l replaced by
-// f_lm alone for odd m at eta = 1/4; S = H_eff for even l + m, v pphi for odd.
+// EOBGetSpinFactorizedWaveform (src/EOB.cu:1236-1613) for every (l, m), l = 2..8,
+// m = 1..l: h_lm = T_lm e^{i delta_lm} S_lm (rho_lm^l + f_lm) h^N_lm, with rho^l
+// replaced by f_lm alone for odd m at eta = 1/4; S = H_eff for even l + m, v pphi
+// for odd; h^N = prefix vPhi2^((l+eps)/2) Y_{l-eps}^{-m}(pi/2, phi).  Each mode's
+// delta, rho and f are the nested polynomials of the reference, term for term.
 static cplx hlm_mode(int l, int m, double r, double phi, double pphi, double v, double Hreal, const FacCoeffs& hc, cplx prefix,
                      double eta, double vPhiInput, double h22_calib) {
-    if (l == 2 && m == 2) return hlm22(r, phi, v, Hreal, hc, prefix, eta, vPhiInput, h22_calib);
     if (eta > 0.25 && eta < 0.25 + 1e-4) eta = 0.25;
     const cplx I(0.0, 1.0);
     const double v2 = v * v, Omega = v2 * v, vh3 = Hreal * Omega, vh = std::pow(vh3, 1. / 3.);
@@ -650,44 +650,116 @@ static cplx hlm_mode(int l, int m, double r, double phi, double pphi, double v, 
     vPhi *= Omega;
     const double vPhi2 = vPhi * vPhi;
     const int eps = (l + m) % 2;
-    cplx hN = prefix * std::pow(vPhi2, (l + eps) / 2.0) * ylm_equator(l - eps, -m, phi);
+    const cplx hN = prefix * std::pow(vPhi2, (l + eps) / 2.0) * ylm_equator(l - eps, -m, phi);
     const double Slm = (eps == 0) ? (Hreal * Hreal - 1.0) / (2.0 * eta) + 1.0 : v * pphi;
     const double k = m * Omega, hk = Hreal * k;
     const double z2 = std::tgamma((double)(l + 1));
     cplx Tlm = std::exp(cplx(PI * hk, 2.0 * hk * std::log(4.0 * k / std::sqrt(std::exp(1.0))))) * cgamma(cplx(l + 1.0, -2.0 * hk));
     Tlm /= z2;
-    double delta = 0.0, rho = 1.0;
+    const FacCoeffs& c = hc;
+    double d = 0.0, rho = 1.0;
     cplx f = 0.0;
-    if (l == 2 && m == 1) {
-        delta = vh3 * (hc.delta21vh3 + vh3 * (hc.delta21vh6 + vh * (hc.delta21vh7 + hc.delta21vh9 * vh * vh))) + hc.delta21v5 * v * v2 * v2
-                + hc.delta21v7 * v2 * v2 * v2 * v;
-        rho = 1. + v * (hc.rho21v1 + v * (hc.rho21v2 + v * (hc.rho21v3 + v * (hc.rho21v4 + v * (hc.rho21v5 + v * (hc.rho21v6
-              + hc.rho21v6l * el + v * (hc.rho21v7 + hc.rho21v7l * el + v * (hc.rho21v8 + hc.rho21v8l * el
-              + (hc.rho21v10 + hc.rho21v10l * el) * v2))))))));
-        f = v * (hc.f21v1 + v2 * (hc.f21v3 + v * hc.f21v4 + v2 * (hc.f21v5 + v * hc.f21v6 + v2 * hc.f21v7c)));
-    } else if (l == 3 && m == 3) {
-        delta = vh3 * (hc.delta33vh3 + vh3 * (hc.delta33vh6 + hc.delta33vh9 * vh3)) + hc.delta33v5 * v * v2 * v2;
-        rho = 1. + v2 * (hc.rho33v2 + v * (hc.rho33v3 + v * (hc.rho33v4 + v * (hc.rho33v5 + v * (hc.rho33v6 + hc.rho33v6l * el
-              + v * (hc.rho33v7 + v * (hc.rho33v8 + hc.rho33v8l * el + v2 * (hc.rho33v10 + hc.rho33v10l * el))))))));
-        f = v * (v2 * (hc.f33v3 + v * (hc.f33v4 + v * (hc.f33v5 + v * hc.f33v6)))) + I * vh3 * vh3 * hc.f33vh6;
-    } else if (l == 4 && m == 4) {
-        delta = vh3 * (hc.delta44vh3 + vh3 * (hc.delta44vh6 + vh3 * hc.delta44vh9)) + hc.delta44v5 * v2 * v2 * v;
-        rho = 1. + v2 * (hc.rho44v2 + v * (hc.rho44v3 + v * (hc.rho44v4 + v * (hc.rho44v5 + v * (hc.rho44v6 + hc.rho44v6l * el
-              + v2 * (hc.rho44v8 + hc.rho44v8l * el + v2 * (hc.rho44v10 + hc.rho44v10l * el)))))));
-    } else if (l == 5 && m == 5) {
-        delta = vh3 * (hc.delta55vh3 + vh3 * (hc.delta55vh6 + vh3 * hc.delta55vh9)) + hc.delta55v5 * v2 * v2 * v;
-        rho = 1. + v2 * (hc.rho55v2 + v * (hc.rho55v3 + v * (hc.rho55v4 + v * (hc.rho55v5 + v * (hc.rho55v6 + hc.rho55v6l * el
-              + v2 * (hc.rho55v8 + hc.rho55v8l * el + v2 * (hc.rho55v10 + hc.rho55v10l * el)))))));
-        f = v2 * v * (hc.f55v3 + v * (hc.f55v4 + v * hc.f55v5c));
-    } else {
-        return cplx(NAN, NAN);
+    const int lm = 10 * l + m;
+    switch (lm) {
+    case 22:
+        d = vh3 * (c.delta22vh3 + vh3 * (c.delta22vh6 + vh * vh * (c.delta22vh9 * vh))) + c.delta22v5 * v * v2 * v2 + c.delta22v6 * v2 * v2 * v2
+            + c.delta22v8 * v2 * v2 * v2 * v2;
+        rho = 1. + v2 * (c.rho22v2 + v * (c.rho22v3 + v * (c.rho22v4 + v * (c.rho22v5 + v * (c.rho22v6 + c.rho22v6l * el + v * (c.rho22v7
+              + v * (c.rho22v8 + c.rho22v8l * el + (c.rho22v10 + c.rho22v10l * el) * v2))))))) + h22_calib * std::pow(v, 12);
+        break;
+    case 21:
+        d = vh3 * (c.delta21vh3 + vh3 * (c.delta21vh6 + vh * (c.delta21vh7 + c.delta21vh9 * vh * vh))) + c.delta21v5 * v * v2 * v2
+            + c.delta21v7 * v2 * v2 * v2 * v;
+        rho = 1. + v * (c.rho21v1 + v * (c.rho21v2 + v * (c.rho21v3 + v * (c.rho21v4 + v * (c.rho21v5 + v * (c.rho21v6 + c.rho21v6l * el
+              + v * (c.rho21v7 + c.rho21v7l * el + v * (c.rho21v8 + c.rho21v8l * el + (c.rho21v10 + c.rho21v10l * el) * v2))))))));
+        f = v * (c.f21v1 + v2 * (c.f21v3 + v * c.f21v4 + v2 * (c.f21v5 + v * c.f21v6 + v2 * c.f21v7c)));
+        break;
+    case 33:
+        d = vh3 * (c.delta33vh3 + vh3 * (c.delta33vh6 + c.delta33vh9 * vh3)) + c.delta33v5 * v * v2 * v2;
+        rho = 1. + v2 * (c.rho33v2 + v * (c.rho33v3 + v * (c.rho33v4 + v * (c.rho33v5 + v * (c.rho33v6 + c.rho33v6l * el
+              + v * (c.rho33v7 + v * (c.rho33v8 + c.rho33v8l * el + v2 * (c.rho33v10 + c.rho33v10l * el))))))));
+        f = v * (v2 * (c.f33v3 + v * (c.f33v4 + v * (c.f33v5 + v * c.f33v6)))) + I * vh3 * vh3 * c.f33vh6;
+        break;
+    case 32:
+        d = vh3 * (c.delta32vh3 + vh * (c.delta32vh4 + vh * vh * (c.delta32vh6 + c.delta32vh9 * vh3)));
+        rho = 1. + v * (c.rho32v + v * (c.rho32v2 + v * (c.rho32v3 + v * (c.rho32v4 + v * (c.rho32v5 + v * (c.rho32v6 + c.rho32v6l * el
+              + (c.rho32v8 + c.rho32v8l * el) * v2))))));
+        break;
+    case 31:
+        d = vh3 * (c.delta31vh3 + vh3 * (c.delta31vh6 + vh * (c.delta31vh7 + c.delta31vh9 * vh * vh))) + c.delta31v5 * v * v2 * v2;
+        rho = 1. + v2 * (c.rho31v2 + v * (c.rho31v3 + v * (c.rho31v4 + v * (c.rho31v5 + v * (c.rho31v6 + c.rho31v6l * el
+              + v * (c.rho31v7 + (c.rho31v8 + c.rho31v8l * el) * v))))));
+        f = v * v2 * c.f31v3;
+        break;
+    case 44:
+        d = vh3 * (c.delta44vh3 + vh3 * (c.delta44vh6 + vh3 * c.delta44vh9)) + c.delta44v5 * v2 * v2 * v;
+        rho = 1. + v2 * (c.rho44v2 + v * (c.rho44v3 + v * (c.rho44v4 + v * (c.rho44v5 + v * (c.rho44v6 + c.rho44v6l * el
+              + v2 * (c.rho44v8 + c.rho44v8l * el + v2 * (c.rho44v10 + c.rho44v10l * el)))))));
+        break;
+    case 43:
+        d = vh3 * (c.delta43vh3 + vh * (c.delta43vh4 + c.delta43vh6 * vh * vh));
+        rho = 1. + v * (c.rho43v + v * (c.rho43v2 + v2 * (c.rho43v4 + v * (c.rho43v5 + (c.rho43v6 + c.rho43v6l * el) * v))));
+        f = v * c.f43v;
+        break;
+    case 42:
+        d = vh3 * (c.delta42vh3 + c.delta42vh6 * vh3);
+        rho = 1. + v2 * (c.rho42v2 + v * (c.rho42v3 + v * (c.rho42v4 + v * (c.rho42v5 + (c.rho42v6 + c.rho42v6l * el) * v))));
+        break;
+    case 41:
+        d = vh3 * (c.delta41vh3 + vh * (c.delta41vh4 + c.delta41vh6 * vh * vh));
+        rho = 1. + v * (c.rho41v + v * (c.rho41v2 + v2 * (c.rho41v4 + v * (c.rho41v5 + (c.rho41v6 + c.rho41v6l * el) * v))));
+        f = v * c.f41v;
+        break;
+    case 55:
+        d = vh3 * (c.delta55vh3 + vh3 * (c.delta55vh6 + vh3 * c.delta55vh9)) + c.delta55v5 * v2 * v2 * v;
+        rho = 1. + v2 * (c.rho55v2 + v * (c.rho55v3 + v * (c.rho55v4 + v * (c.rho55v5 + v * (c.rho55v6 + c.rho55v6l * el
+              + v2 * (c.rho55v8 + c.rho55v8l * el + v2 * (c.rho55v10 + c.rho55v10l * el)))))));
+        f = v2 * v * (c.f55v3 + v * (c.f55v4 + v * c.f55v5c));
+        break;
+    case 54: d = vh3 * (c.delta54vh3 + c.delta54vh4 * vh); rho = 1. + v2 * (c.rho54v2 + v * (c.rho54v3 + c.rho54v4 * v)); break;
+    case 53: d = c.delta53vh3 * vh3; rho = 1. + v2 * (c.rho53v2 + v * (c.rho53v3 + v * (c.rho53v4 + c.rho53v5 * v))); break;
+    case 52: d = vh3 * (c.delta52vh3 + c.delta52vh4 * vh); rho = 1. + v2 * (c.rho52v2 + v * (c.rho52v3 + c.rho52v4 * v)); break;
+    case 51: d = c.delta51vh3 * vh3; rho = 1. + v2 * (c.rho51v2 + v * (c.rho51v3 + v * (c.rho51v4 + c.rho51v5 * v))); break;
+    case 66: d = c.delta66vh3 * vh3; rho = 1. + v2 * (c.rho66v2 + v * (c.rho66v3 + c.rho66v4 * v)); break;
+    case 65: d = c.delta65vh3 * vh3; rho = 1. + v2 * (c.rho65v2 + c.rho65v3 * v); break;
+    case 64: d = c.delta64vh3 * vh3; rho = 1. + v2 * (c.rho64v2 + v * (c.rho64v3 + c.rho64v4 * v)); break;
+    case 63: d = c.delta63vh3 * vh3; rho = 1. + v2 * (c.rho63v2 + c.rho63v3 * v); break;
+    case 62: d = c.delta62vh3 * vh3; rho = 1. + v2 * (c.rho62v2 + v * (c.rho62v3 + c.rho62v4 * v)); break;
+    case 61: d = c.delta61vh3 * vh3; rho = 1. + v2 * (c.rho61v2 + c.rho61v3 * v); break;
+    case 77: d = c.delta77vh3 * vh3; rho = 1. + v2 * (c.rho77v2 + c.rho77v3 * v); break;
+    case 76: rho = 1. + c.rho76v2 * v2; break;
+    case 75: d = c.delta75vh3 * vh3; rho = 1. + v2 * (c.rho75v2 + c.rho75v3 * v); break;
+    case 74: rho = 1. + c.rho74v2 * v2; break;
+    case 73: d = c.delta73vh3 * vh3; rho = 1. + v2 * (c.rho73v2 + c.rho73v3 * v); break;
+    case 72: rho = 1. + c.rho72v2 * v2; break;
+    case 71: d = c.delta71vh3 * vh3; rho = 1. + v2 * (c.rho71v2 + c.rho71v3 * v); break;
+    case 88: rho = 1. + c.rho88v2 * v2; break;
+    case 87: rho = 1. + c.rho87v2 * v2; break;
+    case 86: rho = 1. + c.rho86v2 * v2; break;
+    case 85: rho = 1. + c.rho85v2 * v2; break;
+    case 84: rho = 1. + c.rho84v2 * v2; break;
+    case 83: rho = 1. + c.rho83v2 * v2; break;
+    case 82: rho = 1. + c.rho82v2 * v2; break;
+    case 81: rho = 1. + c.rho81v2 * v2; break;
+    default: return cplx(NAN, NAN);
     }
     cplx rpl = (eta == 0.25 && (m % 2)) ? f : cplx(std::pow(rho, l), 0.0) + f;
-    cplx h = Tlm * std::exp(I * delta) * Slm * rpl;
+    cplx h = Tlm * std::exp(I * d) * Slm * rpl;
     return h * hN;
 }
 
-// EOBGetSpinFactorizedWaveform_wrap for (l, m) in {(2,2), (2,1), (3,3), (4,4), (5,5)}:
+// |h_lm|^2 as the flux evaluates it (EOBFluxGetSpinFactorizedWaveform,
+// src/EOB.cu:1706-1941) at npts points of one binary (coefficient record coeffs)
+void orc_flux_mode_abs2(double* out, int l, int m, const double* v, const double* Hreal, const double* pphi, const double* vPhi2,
+                        int npts, double prefix_abs, double eta, double h22_calib, const double* coeffs) {
+    const FacCoeffs& hc = *reinterpret_cast<const FacCoeffs*>(coeffs);
+    for (int i = 0; i < npts; ++i) {
+        const double vv = v[i];
+        out[i] = flux_mode_abs2(hc, l, m, vv, Hreal[i], pphi[i], prefix_abs, eta, vPhi2[i], vv * vv * vv, vv * vv, h22_calib);
+    }
+}
+
+// EOBGetSpinFactorizedWaveform_wrap for any (l, m) with 2 <= l <= 8, 1 <= m <= l:
 // inputs per knot [numSys][traj_length]; prefix complex per binary.
 void orc_hlm(double* out, int l, int m, const double* r, const double* phi, const double* pphi, const double* v, const double* Hreal,
              const double* eta, const double* vPhiIn, const double* prefix, double h22_calib, int numSys, int traj_length,
@@ -742,6 +814,34 @@ void orc_get_hlm22(double* out, const double* traj, const int* lengths, int nbin
             const double Hv = nu * hamiltonian<double>(r, pr, L, p);
             const double v = std::pow(omega, 1. / 3);
             cplx h = hlm22(r, phi, v, Hv, hc[b], pf, nu, vPhi, h22_calib);
+            out[2 * q] = h.real(); out[2 * q + 1] = h.imag();
+        }
+    }
+}
+
+// get_hlms (eob/seobnrv4phm.py:2138-2221) for any (l, m): the mode on every knot of
+// traj [nbin][4][len]; the prefix is additionalArgs' entry of (l, m).
+void orc_get_hlm(double* out, int l, int m, const double* traj, const int* lengths, int nbin, int len, const double* add,
+                 const double* coeffs, double h22_calib) {
+    const FacCoeffs* hc = reinterpret_cast<const FacCoeffs*>(coeffs);
+    const int fi = (l - 1) * l / 2 - 1 + m - 1;   // flux order l = 2..8, m = 1..l
+    for (int b = 0; b < nbin; ++b) {
+        HamParams p = params_of(add, nbin, b);
+        const double M = p.m1 + p.m2, nu = (p.m1 * p.m2 / M) / M;
+        cplx pf(add[(10 + 2 * fi) * nbin + b], add[(11 + 2 * fi) * nbin + b]);
+        for (int s = 0; s < len; ++s) {
+            size_t q = (size_t)b * len + s;
+            if (s >= lengths[b]) { out[2 * q] = 0.0; out[2 * q + 1] = 0.0; continue; }
+            const double r = traj[((size_t)b * 4 + 0) * len + s], phi = traj[((size_t)b * 4 + 1) * len + s];
+            const double pr = traj[((size_t)b * 4 + 2) * len + s], L = traj[((size_t)b * 4 + 3) * len + s];
+            double g[4], gc[4];
+            grad1(r, pr, L, p, g);
+            grad1(r, 0.0, L, p, gc);
+            const double omega = g[3], omega_c = gc[3];
+            const double vPhi = 1 / (omega_c * omega_c * r * r * r);
+            const double Hv = nu * hamiltonian<double>(r, pr, L, p);
+            const double v = std::pow(omega, 1. / 3);
+            cplx h = hlm_mode(l, m, r, phi, L, v, Hv, hc[b], pf, nu, vPhi, h22_calib);
             out[2 * q] = h.real(); out[2 * q + 1] = h.imag();
         }
     }

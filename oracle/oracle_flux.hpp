@@ -51,11 +51,19 @@ struct FacCoeffs {
     double rho77v2, rho77v3, rho76v2, rho75v2, rho75v3, rho74v2, rho73v2, rho73v3, rho72v2, rho71v2, rho71v3;
     // (8,m)
     double rho88v2, rho87v2, rho86v2, rho85v2, rho84v2, rho83v2, rho82v2, rho81v2;
-    // phases of the higher modes the waveform evaluates (2,1), (3,3), (4,4), (5,5)
+    // phases delta_lm of the modes other than (2,2), as the waveform reads them
     double delta21vh3, delta21vh6, delta21vh7, delta21vh9, delta21v5, delta21v7;
     double delta33vh3, delta33vh6, delta33vh9, delta33v5;
+    double delta32vh3, delta32vh4, delta32vh6, delta32vh9;
+    double delta31vh3, delta31vh6, delta31vh7, delta31vh9, delta31v5;
     double delta44vh3, delta44vh6, delta44vh9, delta44v5;
+    double delta43vh3, delta43vh4, delta43vh6;
+    double delta42vh3, delta42vh6;
+    double delta41vh3, delta41vh4, delta41vh6;
     double delta55vh3, delta55vh6, delta55vh9, delta55v5;
+    double delta54vh3, delta54vh4, delta53vh3, delta52vh3, delta52vh4, delta51vh3;
+    double delta66vh3, delta65vh3, delta64vh3, delta63vh3, delta62vh3, delta61vh3;
+    double delta77vh3, delta75vh3, delta73vh3, delta71vh3;
 };
 
 // src/EOB.cu:1946-2462
@@ -364,6 +372,41 @@ inline void calc_fac_coeffs(FacCoeffs* c, int use_hm, double m1, double m2, doub
     c->delta55vh3 = (96875.0 + 857528.0 * eta) / (131250.0 * (1 - 2 * eta));
     c->delta55vh6 = use_hm ? 3865.0 / 429.0 * PI : 0.0;
     c->delta55vh9 = use_hm ? (-7686949127.0 + 954500400.0 * PI * PI) / 31783752.0 : 0.0;
+    // the other modes' phases (src/EOB.cu:2134-2137, 2150-2154, 2213-2215, 2231-2232,
+    // 2243-2245, 2317-2341, 2353-2424), aDelta = 0
+    c->delta32vh3 = (10.0 + 33.0 * eta) / (-15.0 * m1p3);
+    c->delta32vh4 = 4.0 * aDelta;
+    c->delta32vh6 = (52.0 * PI) / 21.0;
+    c->delta32vh9 = -9112.0 / 405.0 + (208.0 * PI * PI) / 63.0;
+    c->delta31vh3 = 13.0 / 30.0;
+    c->delta31vh6 = (13.0 * PI) / 21.0;
+    c->delta31vh7 = 0.0;
+    c->delta31vh9 = -227827.0 / 81000.0 + (26.0 * PI * PI) / 63.0;
+    c->delta31v5 = -17.0 * eta / 10.0;
+    c->delta43vh3 = (486.0 + 4961.0 * eta) / (810.0 * (1.0 - 2.0 * eta));
+    c->delta43vh4 = 0.0;
+    c->delta43vh6 = 1571.0 * PI / 385.0;
+    c->delta42vh3 = (7.0 * (1.0 + 6.0 * eta)) / (-15.0 * m1p3);
+    c->delta42vh6 = (6284.0 * PI) / 3465.0;
+    c->delta41vh3 = (2.0 + 507.0 * eta) / (10.0 * (1.0 - 2.0 * eta));
+    c->delta41vh4 = 0.0;
+    c->delta41vh6 = 1571.0 * PI / 3465.0;
+    c->delta54vh3 = 8.0 / 15.0;
+    c->delta54vh4 = 0.0;
+    c->delta53vh3 = 31.0 / 70.0;
+    c->delta52vh3 = 4.0 / 15.0;
+    c->delta52vh4 = 0.0;
+    c->delta51vh3 = 31.0 / 210.0;
+    c->delta66vh3 = 43.0 / 70.0;
+    c->delta65vh3 = 10.0 / 21.0;
+    c->delta64vh3 = 43.0 / 105.0;
+    c->delta63vh3 = 2.0 / 7.0;
+    c->delta62vh3 = 43.0 / 210.0;
+    c->delta61vh3 = 2.0 / 21.0;
+    c->delta77vh3 = 19.0 / 36.0;
+    c->delta75vh3 = 95.0 / 252.0;
+    c->delta73vh3 = 19.0 / 84.0;
+    c->delta71vh3 = 19.0 / 252.0;
 }
 
 // |Y_{l}^{-m}(theta=pi/2, phi=0)| for 0<=m<=l<=8 from the closed form

@@ -1,6 +1,7 @@
 """GPU: detector projection (antenna patterns, delays, detector-frame signal,
 eob/seobnrv4phm.py:597-634, 742-767) against the oracle's numpy restatement,
-and the two-polarization resample (h+ - i hx = amp h22) against the h+ resample
+and the two-polarization resample (h+ + i hx = amp h22: the real and imaginary
+parts of the template, as the reference transforms them) against the h+ resample
 and scipy's splines."""
 import numpy as np
 import pytest
@@ -32,7 +33,9 @@ def test_detector_response_and_signal(oracle, dev):
     Fp_r, Fc_r, dt_r = oracle.detector_information(ra, dec, gmst, psi, D, V)
     np.testing.assert_allclose(Fp.cpu().numpy(), Fp_r, rtol=1e-13, atol=1e-15)
     np.testing.assert_allclose(Fc.cpu().numpy(), Fc_r, rtol=1e-13, atol=1e-15)
-    np.testing.assert_allclose(dt.cpu().numpy(), dt_r, rtol=1e-13, atol=1e-19)
+    # the delay is a dot product (unit vector . vertex / c) that cancels for
+    # directions across the vertex: bounded relative to the delays' scale
+    np.testing.assert_allclose(dt.cpu().numpy(), dt_r, rtol=1e-13, atol=1e-13 * np.abs(dt_r).max())
     F, df, kmin = 4097, 0.25, 80
     nb = 7
     hp = rng.normal(size=(nb, F)) + 1j * rng.normal(size=(nb, F))
@@ -69,5 +72,5 @@ def test_two_polarization_resample(dev):
         h22 = 2.0 * (s[1] + 1j * s[2]) * np.exp(-2j * s[0])
         scale = np.abs(h22).max()
         assert np.max(np.abs(hp.cpu().numpy()[i, :ns] - h22.real)) < 1e-10 * scale
-        assert np.max(np.abs(hc.cpu().numpy()[i, :ns] + h22.imag)) < 1e-10 * scale
+        assert np.max(np.abs(hc.cpu().numpy()[i, :ns] - h22.imag)) < 1e-10 * scale
         assert not hc[i, ns:].any()

@@ -307,3 +307,34 @@ def test_initial_condition_solvers(oracle, dev):
     np.testing.assert_allclose(F.cpu().numpy(), F_r, rtol=5e-10, atol=1e-20)
     np.testing.assert_allclose(gt.cpu().numpy()[[0, 3]], gt_r[[0, 3]], rtol=2e-10, atol=1e-16)
     ode.deallocate_information()
+
+
+def test_stopping_criterion_codes_2_to_4(oracle, dev):
+    """Stop codes 2-4 of StoppingCriterion.__call__ (eob/seobnrv4phm.py:109-117)
+    on chosen states, with the expected code read off the oracle's right-hand
+    side: r < 1 gives 2 whatever the derivatives (no trajectory reaches it: the
+    Hamiltonian leaves its domain above r = 1 for every spin, and the run ends
+    with -1 first); r < 6 with dr/dt > 0 gives 3; r < 6 with dpr/dt > 0 and
+    dr/dt <= 0 gives 4; r >= 6 gives 0 (step numbers <= 5: no omega check)."""
+    n = 8
+    add, co, cond = _start(oracle, n, 20.0)
+    r = np.array([0.5, 0.8, 5.0, 4.0, 5.0, 4.0, 8.0, 12.0])
+    L = np.sqrt(r) * np.array([1.0, 1.0, 1.0, 1.0, 3.0, 3.0, 1.0, 1.0])
+    pr = np.array([-0.1, 0.0, 0.2, 0.1, -0.01, -0.01, 0.1, -0.1])
+    st = np.array([r, np.zeros(n), pr, L])
+    k = oracle.rhs(st, add, co)
+    exp = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        if r[i] < 1.0:
+            exp[i] = 2
+        elif r[i] < 6.0 and k[0, i] > 0:
+            exp[i] = 3
+        elif r[i] < 6.0 and k[2, i] > 0:
+            exp[i] = 4
+    assert set(exp.tolist()) >= {0, 2, 3, 4}, (exp, k)
+    ode = _ode(add, dev)
+    sc = StoppingCriterion(ode, 64)
+    sc.setup(n)
+    stop = sc(torch.zeros(n, dtype=torch.int64, device=dev), _T(st, dev), _T(add, dev), torch.arange(n, device=dev))
+    np.testing.assert_array_equal(stop.cpu().numpy(), exp)
+    ode.deallocate_information()

@@ -359,6 +359,97 @@ def test_h22_on_knots_against_factorized_form(oracle):
         np.testing.assert_allclose(h_o, h, rtol=1e-12)
 
 
+def _mode_points(rng, nu, L=24):
+    v = rng.uniform(0.15, 0.45, L)
+    Hr = 1.0 - nu * v ** 2 * rng.uniform(0.3, 0.6, L)
+    r = v ** -2 * rng.uniform(0.95, 1.05, L)
+    vPhiIn = rng.uniform(0.9, 1.1, L) / (v ** 6 * r ** 3)
+    pphi = np.sqrt(r) * rng.uniform(0.98, 1.1, L)
+    return v, Hr, r, vPhiIn, pphi
+
+
+ALL_MODES = [(l, m) for l in range(2, 9) for m in range(1, l + 1)]
+
+
+def test_higher_mode_moduli_equal_the_flux_modes(oracle):
+    """|h_lm| of the waveform (complex Gamma tail, nested rho and f, Y_lm with
+    its phase) equals |h_lm| as the flux computes it (the product form of |T_lm|,
+    the flux's own rho/f polynomials, |Y_lm(pi/2)|) for all 35 modes where the
+    two polynomials coincide (use_hm = 0); with use_hm = 1 the waveform's extra
+    terms of (2,1), (3,3), (4,4), (5,5) make those four differ and no other."""
+    rng = np.random.default_rng(41)
+    for m1, m2, c1, c2 in ((0.7, 0.3, 0.4, -0.2), (0.55, 0.45, -0.6, 0.5), (0.5, 0.5, 0.3, -0.1)):
+        add = oracle.make_additional_args(np.array([m1]), np.array([m2]), np.array([c1]), np.array([c2]))
+        nu = m1 * m2 / (m1 + m2) ** 2
+        v, Hr, r, vPhiIn, pphi = _mode_points(rng, nu)
+        phi = rng.uniform(0.0, 40.0, v.size)
+        vPhi2 = (r * np.cbrt(vPhiIn) * v ** 3) ** 2
+        for use_hm in (0, 1):
+            co = oracle.fac_coeffs(add, use_hm=use_hm)
+            for q, (l, m) in enumerate(ALL_MODES):
+                pf = add[10 + 2 * q, 0] + 1j * add[11 + 2 * q, 0]
+                h = oracle.hlm(l, m, r[None], phi[None], pphi[None], v[None], Hr[None], np.array([nu]), vPhiIn[None], np.array([pf]), co)[0]
+                a2 = oracle.flux_mode_abs2(l, m, v, Hr, pphi, vPhi2, abs(pf), nu, co[0])
+                close = np.allclose(np.abs(h) ** 2, a2, rtol=1e-12, atol=0)
+                if use_hm == 0:
+                    assert close, ((m1, m2), (l, m))
+                else:
+                    extra = (l, m) in ((2, 1), (3, 3), (4, 4), (5, 5))
+                    assert close != extra, ((m1, m2), (l, m))
+
+
+def test_higher_mode_tail_and_multipole_phase(oracle):
+    """The phases the modes share: for the modes with no delta_lm and no f_lm
+    ((7, even m) and l = 8), arg h_lm at phi = 0 is arg T_lm + arg prefix (mod pi)
+    with arg T_lm = 2 k ln(4 m Omega / sqrt(e)) + Im lnGamma(l + 1 - 2 i k),
+    k = H m Omega, from scipy's loggamma; and every mode turns as e^{-i m phi}."""
+    rng = np.random.default_rng(43)
+    add = oracle.make_additional_args(np.array([0.62]), np.array([0.38]), np.array([0.3]), np.array([0.1]))
+    co = oracle.fac_coeffs(add)
+    nu = 0.62 * 0.38
+    v, Hr, r, vPhiIn, pphi = _mode_points(rng, nu)
+    for q, (l, m) in enumerate(ALL_MODES):
+        pf = add[10 + 2 * q, 0] + 1j * add[11 + 2 * q, 0]
+
+        def h_at(phi):
+            return oracle.hlm(l, m, r[None], np.full((1, v.size), phi), pphi[None], v[None], Hr[None], np.array([nu]), vPhiIn[None],
+                              np.array([pf]), co)[0]
+        h0 = h_at(0.0)
+        phi1 = 0.7
+        np.testing.assert_allclose(h_at(phi1), h0 * np.exp(-1j * m * phi1), rtol=1e-12)
+        if (l == 7 and m % 2 == 0) or l == 8:
+            k = Hr * m * v ** 3
+            argT = 2 * k * np.log(4 * m * v ** 3 / math.sqrt(math.e)) + sps.loggamma(l + 1 - 2j * k).imag
+            d = np.angle(h0 / pf) - argT
+            d = (d + math.pi / 2) % math.pi - math.pi / 2     # mod pi (the sign of Y_lm and rho^l)
+            assert np.max(np.abs(d)) < 1e-11, (l, m)
+
+
+def test_higher_mode_leading_phases_and_22_consistency(oracle):
+    """delta_lm's leading (H Omega) coefficients of the four higher modes against
+    Damour-Iyer-Nagar 2009 (delta_21 = 2/3, delta_33 = 13/10, delta_44 = (112 +
+    219 nu) / (120 (1 - 3 nu)), delta_55 = (96875 + 857528 nu) / (131250 (1 - 2 nu))),
+    and the general waveform's (2,2) equals the (2,2) evaluation."""
+    for nu_m1 in (0.7, 0.55):
+        m1, m2 = nu_m1, 1 - nu_m1
+        add = oracle.make_additional_args(np.array([m1]), np.array([m2]), np.array([0.2]), np.array([-0.3]))
+        nu = m1 * m2
+        c = oracle.fac_coeffs_named(add)[0]
+        assert c["delta21vh3"] == pytest.approx(2.0 / 3.0, rel=1e-15)
+        assert c["delta33vh3"] == pytest.approx(13.0 / 10.0, rel=1e-15)
+        assert c["delta44vh3"] == pytest.approx((112 + 219 * nu) / (120 * (1 - 3 * nu)), rel=1e-14)
+        assert c["delta55vh3"] == pytest.approx((96875 + 857528 * nu) / (131250 * (1 - 2 * nu)), rel=1e-14)
+        co = oracle.fac_coeffs(add)
+        rng = np.random.default_rng(47)
+        v, Hr, r, vPhiIn, pphi = _mode_points(rng, nu)
+        phi = rng.uniform(0.0, 40.0, v.size)
+        pf = np.array([add[12, 0] + 1j * add[13, 0]])
+        for h22c in (0.0, 0.3):
+            a = oracle.hlm(2, 2, r[None], phi[None], pphi[None], v[None], Hr[None], np.array([nu]), vPhiIn[None], pf, co, h22_calib=h22c)
+            b = oracle.hlm22(r[None], phi[None], v[None], Hr[None], np.array([nu]), vPhiIn[None], pf, co, h22c)
+            np.testing.assert_allclose(a, b, rtol=1e-14)
+
+
 # ------------------------------------------------------ initial conditions
 def test_initial_condition_residuals(oracle):
     """The circular start solves dH/dr = 0 at pr = 0 for pphi0, and the
@@ -390,6 +481,70 @@ def test_initial_condition_residuals(oracle):
     assert np.all(np.abs(resid) <= 1e-9 * np.abs(g[2])), resid / g[2]
     np.testing.assert_allclose(om0, g0[3], rtol=1e-14)
     assert np.all(pr0 < 0) and np.all(pr0 > -3e-2)
+
+
+def test_initial_condition_drop_in_solvers(oracle):
+    """The reference's solvers restated in the oracle: root_find_all's (r, pphi)
+    solve IC_cons (dH/dpphi = omega0 and dH/dr = 0 at pr = 0, src/EOB.cu:2840-3044)
+    from a perturbed start; root_find_scalar_all's pr solves IC_diss's dissipative
+    condition (src/EOB.cu:2691-2838) -- both checked with the separately pinned
+    gradient, Hessian and radiation-reaction force."""
+    rng = np.random.default_rng(45)
+    n = 10
+    q = rng.uniform(1, 8, n)
+    add = oracle.make_additional_args(q / (1 + q), 1 / (1 + q), rng.uniform(-0.8, 0.8, n), rng.uniform(-0.8, 0.8, n))
+    co = oracle.fac_coeffs(add)
+    r0 = rng.uniform(12.0, 30.0, n)
+    pphi0, pr0, om0, add2 = oracle.initial_conditions(r0, add, co)
+    x0 = np.array([r0 * rng.uniform(0.97, 1.03, n), pphi0 * rng.uniform(0.98, 1.02, n)])
+    x = oracle.root_find_all(x0, add2)
+    st = np.array([x[0], np.zeros(n), np.zeros(n), x[1]])
+    g = oracle.grad_H(st, add2)
+    assert np.all(np.abs(g[3] - add2[4]) <= 1e-12 * add2[4]), g[3] - add2[4]
+    assert np.all(np.abs(g[0]) <= 1e-12 * g[3] / x[0]), g[0]
+    np.testing.assert_allclose(x[0], r0, rtol=1e-9)
+    np.testing.assert_allclose(x[1], pphi0, rtol=1e-9)
+    args = np.array([r0, np.zeros(n), np.zeros(n), pphi0])
+    bounds = np.array([np.full(n, -3e-2), np.zeros(n)])
+    pr = oracle.root_find_scalar_all(bounds, args, add2, co)
+    s2 = np.array([r0, np.zeros(n), pr, pphi0])
+    g2 = oracle.grad_H(s2, add2)
+    hs = oracle.hessian_H(s2, add2)
+    aw = add2.copy()
+    aw[4] = g2[3]
+    F = oracle.rr_force(s2, aw, co)
+    resid = F[1] / (-hs[0, 0] / hs[3, 0]) - g2[2]
+    assert np.all(np.abs(resid) < 1e-12), resid
+    out = oracle.ic_diss(pr, args, add2, co)[0]
+    assert np.all(np.abs(out) < 1e-12), out
+
+
+def test_dop853_attempt_stages(oracle):
+    """One attempted step of the oracle's stepper equals the stages formed here
+    from scipy's DOP853 tableau and the oracle's (separately pinned) right-hand
+    side: k_s = f(y + h sum_j a_sj k_j), y_new = y + h sum_s b_s k_s."""
+    import scipy.integrate._ivp.dop853_coefficients as d
+    rng = np.random.default_rng(49)
+    n = 6
+    q = rng.uniform(1, 6, n)
+    add = oracle.make_additional_args(q / (1 + q), 1 / (1 + q), rng.uniform(-0.5, 0.5, n), rng.uniform(-0.5, 0.5, n))
+    co = oracle.fac_coeffs(add)
+    r0 = rng.uniform(8.0, 20.0, n)
+    pphi0, pr0, om0, add = oracle.initial_conditions(r0, add, co)
+    y = np.array([r0, rng.uniform(0, 10, n), pr0, pphi0])
+    h = rng.uniform(0.5, 4.0, n)
+    k_o, yn_o, err_o = oracle.dopr_attempt(y, h, add, co)
+    k = np.zeros((12, 4, n))
+    for s_ in range(12):
+        acc = np.zeros((4, n))
+        for j in range(s_):
+            acc = acc + d.A[s_, j] * k[j]
+        k[s_] = oracle.rhs(y + h * acc, add, co)
+    yn = y + h * np.tensordot(d.B[:12], k, axes=1)
+    np.testing.assert_allclose(k_o, k, rtol=1e-13, atol=1e-30)
+    np.testing.assert_allclose(yn_o, yn, rtol=1e-14)
+    e_r, _ = oracle.error(y, h, {"k1": k[0], "k2": k[10], "k3": k[11], "k6": k[5], "k7": k[6], "k8": k[7], "k9": k[8], "k10": k[9]})
+    np.testing.assert_allclose(err_o, e_r, rtol=1e-12)
 
 
 # ------------------------------------------------------ special functions
