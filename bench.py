@@ -218,29 +218,37 @@ def time_streamed(pipe, steps, warmup, group=64, block=64, grid_groups=0):
     torch.cuda.synchronize()
     t_traj = t_ing = 0.0
     attempts = []
+    per_step = []
+    evs = [{"start": torch.cuda.Event(enable_timing=True), "traj_end_longest": torch.cuda.Event(enable_timing=True),
+            "end": torch.cuda.Event(enable_timing=True)} for _ in range(steps)]
     t0 = time.perf_counter()
     for k in range(steps):
-        ev = {"start": torch.cuda.Event(enable_timing=True), "traj_end_longest": torch.cuda.Event(enable_timing=True)}
+        ev = evs[k]
+        tk = time.perf_counter()
         ev["start"].record()
-        ti = time.perf_counter()
         ingest(props[warmup + k])
-        t_ing += time.perf_counter() - ti
+        t_ing += time.perf_counter() - tk
         sp.run(group=group, block=block, grid_groups=grid_groups, events=ev)
+        ev["end"].record()
         torch.cuda.synchronize()
-        t_traj += ev["start"].elapsed_time(ev["traj_end_longest"])
-        attempts.append(int(sp.batch.attempts.sum().item()))
+        per_step.append((time.perf_counter() - tk) * 1e3)
     wall = (time.perf_counter() - t0) / steps
-    return wall, logl_same, t_traj / steps, len(sp.groups), t_ing / steps * 1e3, attempts
+    for k in range(steps):
+        t_traj += evs[k]["start"].elapsed_time(evs[k]["traj_end_longest"])
+    gpu_ms = [evs[k]["start"].elapsed_time(evs[k]["end"]) for k in range(steps)]
+    attempts.append(int(sp.batch.attempts.sum().item()))
+    return wall, logl_same, t_traj / steps, len(sp.groups), t_ing / steps * 1e3, attempts, per_step, gpu_ms
 
 
 def time_sustained(pipe, batches, steps, warmup, group=64, block=256):
     """Back-to-back batches: `batches` fresh sets of n binaries (seeds 2000 +
-    batches k + b) in one StreamedPass whose work queue takes them batch by
-    batch (each batch longest expected first), so that the SIMDs freed by a
-    batch's short binaries start the next batch's binaries.  Each binary's log L
-    is its single-batch value (binaries are independent).  One integrator
-    wavefront per SIMD (grid 1024 groups) and two (2048) are timed.  Returns
-    {grid_groups: (wall per pass, evals/s)} and the ingestion time."""
+    batches k + j) in one StreamedPass, so that the SIMDs freed by short
+    binaries start other binaries instead of idling.  Two queue orders: batch
+    by batch (each batch longest expected first; the next batch's binaries
+    fill the SIMDs the previous one frees) and longest expected first over all
+    of them (the batches given together).  Each binary's log L is its
+    single-batch value (binaries are independent).  Returns {order: (wall per
+    pass, evals/s)} and the ingestion time per pass."""
     torch, P = pipe.torch, pipe.P
     from eobgpu_amd import configs
     n = pipe.n
@@ -248,30 +256,31 @@ def time_sustained(pipe, batches, steps, warmup, group=64, block=256):
     for k in range(warmup + steps):
         ps = [configs.uneven_batch(n=n, seed=2000 + batches * k + j) for j in range(batches)]
         cat = {key: np.concatenate([q[key] for q in ps]) for key in ("m1", "m2", "chi1", "chi2", "r0")}
-        order = np.concatenate([j * n + np.argsort(-P.expected_length(q["r0"], q["m1"], q["m2"], q["chi1"], q["chi2"]), kind="stable")
+        major = np.concatenate([j * n + np.argsort(-P.expected_length(q["r0"], q["m1"], q["m2"], q["chi1"], q["chi2"]), kind="stable")
                                 for j, q in enumerate(ps)])
-        sets.append((cat, order))
-    cat, order = sets[0]
+        glob = np.argsort(-P.expected_length(cat["r0"], cat["m1"], cat["m2"], cat["chi1"], cat["chi2"]), kind="stable")
+        sets.append((cat, {"batch_major": major, "longest_first": glob}))
+    cat, orders = sets[0]
     sp = P.StreamedPass(cat["m1"], cat["m2"], cat["chi1"], cat["chi2"], cat["r0"], pipe.lik, cat["m1"] + cat["m2"], pipe.dt,
-                        max_step=pipe.b.max_step, rtol=pipe.rtol, order=order)
+                        max_step=pipe.b.max_step, rtol=pipe.rtol, order=orders["batch_major"])
     out = {}
     t_ing = 0.0
-    for grid in (1024, 2048):
+    for name in ("batch_major", "longest_first"):
         for k in range(warmup):
             c, o = sets[k]
-            sp.set_params(c["m1"], c["m2"], c["chi1"], c["chi2"], c["r0"], order=o)
-            sp.run(group=group, block=block, grid_groups=grid)
+            sp.set_params(c["m1"], c["m2"], c["chi1"], c["chi2"], c["r0"], order=o[name])
+            sp.run(group=group, block=block)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for k in range(steps):
             c, o = sets[warmup + k]
             ti = time.perf_counter()
-            sp.set_params(c["m1"], c["m2"], c["chi1"], c["chi2"], c["r0"], order=o)
+            sp.set_params(c["m1"], c["m2"], c["chi1"], c["chi2"], c["r0"], order=o[name])
             t_ing += time.perf_counter() - ti
-            sp.run(group=group, block=block, grid_groups=grid)
+            sp.run(group=group, block=block)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / steps
-        out[grid] = (wall, batches * n / wall)
+        out[name] = (wall, batches * n / wall)
     return out, t_ing / (2 * steps) * 1e3
 
 
@@ -511,14 +520,15 @@ def workload_capacity(args):
     print(json.dumps(out))
 
 
-def capacity_served(p, args, budget_gb=48.0, N=1 << 12, dt=1.0 / 2048):
+def capacity_served(p, args, budget_gb=6.0, N=1 << 12, dt=1.0 / 2048):
     """The same 16384 binaries as 64 requests of 256 arriving at once, served
-    with admission by projected footprint: size classes of 1024-8192 trajectory
-    rows under a memory budget, each binary in the smallest class its projected
-    length fits, refilled as slots free up (the full chain per binary,
-    likelihood at N = 2^12).  Reports throughput, the classes' memory against
-    one class sized for the longest binary, and the binaries run again after
-    filling their rows."""
+    under one memory budget in two ways: admission by projected footprint (size
+    classes of 1024-8192 trajectory rows, each binary in the smallest class its
+    projected length fits, classes sized by the projected shares) and uniform
+    slots of the longest class (8192 rows, what admission by slot count needs
+    to be safe).  The full chain per binary, likelihood at N = 2^12; refills as
+    slots free up.  Reports for both: binaries resident at once, throughput, and
+    the binaries run again after filling their rows."""
     import torch
     from eobgpu_amd.serving import Request, Server, classes_for_budget, summarize, projected_rows, assign_classes, SlotPool
     rng = np.random.default_rng(5)
@@ -527,22 +537,29 @@ def capacity_served(p, args, budget_gb=48.0, N=1 << 12, dt=1.0 / 2048):
     rows = (1024, 2048, 4096, 8192)
     proj = projected_rows(p)
     share = np.bincount(assign_classes(proj, rows), minlength=len(rows)).astype(np.float64)
-    classes = classes_for_budget(budget_gb * (1 << 30), N, class_rows=rows, weights=np.maximum(share, 1.0) * np.array(rows))
-    srv = Server(data, df=1.0 / (N * dt), dt=dt, N=N, classes=classes, streams=args.serve_streams)
-    srv.warm()
+    budget = budget_gb * (1 << 30)
     n = p["m1"].size
     reqs = [Request(i, 0.0, {k: v[i * 256:(i + 1) * 256] for k, v in p.items()}) for i in range(n // 256)]
-    srv.run(reqs[:1])   # first use of the kernels and buffers
-    t0 = time.perf_counter()
-    res = srv.run(reqs)
-    wall = time.perf_counter() - t0
-    s = summarize(res)
-    uniform = n * SlotPool.bytes_per_binary_for(rows[-1], N)
-    return {"binaries": int(sum(r.nb for r in res)), "seconds": wall, "binaries_per_s": n / wall, "p50_ms": s["p50_ms"],
-            "p99_ms": s["p99_ms"], "classes": [{"max_step": ms, "capacity": c} for ms, c in classes],
-            "class_share_projected": share.astype(int).tolist(), "pool_bytes": int(srv.memory_bytes()),
-            "uniform_pool_bytes_for_all_at_8192_rows": int(uniform), "reruns": int(srv.reruns),
-            "logl_finite": bool(all(np.isfinite(r.logl).all() for r in res))}
+    out = {"budget_bytes": int(budget), "binaries": n, "class_share_projected": share.astype(int).tolist()}
+    uniform_cap = int(budget // SlotPool.bytes_per_binary_for(rows[-1], N))
+    for name, classes in (("footprint_classes", classes_for_budget(budget, N, class_rows=rows,
+                                                                   weights=np.maximum(share, 1.0) * np.array(rows))),
+                          ("uniform_8192_rows", [(rows[-1], uniform_cap)])):
+        srv = Server(data, df=1.0 / (N * dt), dt=dt, N=N, classes=classes, streams=args.serve_streams)
+        srv.warm()
+        srv.run(reqs[:1])   # first use of the kernels and buffers
+        t0 = time.perf_counter()
+        res = srv.run(reqs)
+        wall = time.perf_counter() - t0
+        s = summarize(res)
+        out[name] = {"classes": [{"max_step": ms, "capacity": c} for ms, c in classes],
+                     "resident_binaries": int(sum(c for _, c in classes)), "pool_bytes": int(srv.memory_bytes()),
+                     "seconds": wall, "binaries_per_s": n / wall, "p50_ms": s["p50_ms"], "p99_ms": s["p99_ms"],
+                     "reruns": int(srv.reruns), "logl_finite": bool(all(np.isfinite(r.logl).all() for r in res))}
+        print(f"[bench] capacity served ({name}): {json.dumps(out[name])}", file=sys.stderr, flush=True)
+        del srv, res
+        torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -601,11 +618,12 @@ def main():
     # the streamed passes before the chunked one: a ChunkedPass timed first left the
     # streamed pass slower in the same process
     if args.pass_kind == "streamed":
-        wall_s, logl_s, traj_s, ngroups, ingest_ms, att_s = time_streamed(pipe, args.steps, args.warmup, group=group, block=block,
-                                                                         grid_groups=grid)
+        wall_s, logl_s, traj_s, ngroups, ingest_ms, att_s, per_step, gpu_ms = time_streamed(pipe, args.steps, args.warmup, group=group,
+                                                                                           block=block, grid_groups=grid)
         streamed = {"ms_per_step": wall_s * 1e3, "trajectory_ms": traj_s, "downstream_groups": ngroups,
                     "host_ingest_ms": ingest_ms, "fresh_batches": True,
-                    "attempted_steps_per_pass": {"min": min(att_s), "max": max(att_s), "mean": float(np.mean(att_s))},
+                    "per_step_ms": [round(x, 3) for x in per_step], "per_step_gpu_ms": [round(x, 3) for x in gpu_ms],
+                    "attempted_steps_last_pass": att_s[-1],
                     "one_batch_ms_per_step": wall * 1e3,
                     "max_rel_diff_logl_vs_one_batch": float(np.max(np.abs(logl_s - logl_support) / np.abs(logl_support)))}
         headline_wall = wall_s
@@ -614,11 +632,9 @@ def main():
     sustained = None
     if args.sustained > 0 and args.pass_kind == "streamed":
         res, ing = time_sustained(pipe, args.sustained, max(1, min(args.steps, 3)), 1, group=group, block=block)
-        best = max(res, key=lambda g: res[g][1])
         sustained = {"batches": args.sustained, "binaries_per_pass": args.sustained * args.n, "fresh_batches": True,
-                     "by_grid_groups": {str(g): {"ms_per_pass": w * 1e3, "evals_per_s": v} for g, (w, v) in res.items()},
-                     "evals_per_s": res[best][1], "grid_groups": best, "host_ingest_ms": ing,
-                     "single_batch_evals_per_s": args.n / headline_wall}
+                     "by_order": {o: {"ms_per_pass": w * 1e3, "evals_per_s": v} for o, (w, v) in res.items()},
+                     "host_ingest_ms": ing, "single_batch_evals_per_s": args.n / headline_wall}
         print(f"[bench] sustained ({args.sustained} batches back to back): {json.dumps(sustained)}", file=sys.stderr, flush=True)
 
     fractions = [float(x) for x in args.chunk_fractions.split(",") if x.strip()]

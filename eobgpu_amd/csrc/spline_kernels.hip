@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_TD(const double* __restr
 // traffic is t and y read once and the coefficients written once (the
 // recomputation and the halos re-read rows from the caches).
 constexpr int kHalo = 64;
-constexpr int kChunkMin = 128;
+constexpr int kChunkMin = 64;
 constexpr int kSplineLanes = 64;
 constexpr int kSeg = 16;
 constexpr int kMaxSeries = 32;

@@ -198,3 +198,37 @@ def test_capacity_workload_properties(dev):
     ok = torch.isfinite(b.dense).all(dim=2) | ~live
     good = si != -1
     assert bool(ok[torch.as_tensor(good, device=dev)].all())
+
+
+def test_plunge_edge_binaries(oracle, dev):
+    """The three binaries of the 1024-binary workload that the oracle's adaptive
+    run ends with -1 (the next stage leaves the Hamiltonian's domain before any
+    stop condition holds; the GPU's adaptive run ends two of them with code 4,
+    the adaptive step sequences of two FP64 builds parting at rounding level
+    there).  With pass 1 at fixed step the sequence is deterministic: identical
+    stored points and stop codes, states 3e-9 (the fixture's bound, see the
+    test above); adaptive, the device's codes are among those the oracle's own
+    one-ulp spread reaches on these binaries: perturbing one initial state
+    component by one ulp moves the oracle's endings between -1 and 4 or 5."""
+    g = _fixture("plunge_edge.json")
+    idx = np.array(g["index"])
+    p = configs.uneven_batch(n=1024, seed=2)
+    add = P.make_additional_args(p["m1"][idx], p["m2"][idx], p["chi1"][idx], p["chi2"][idx])
+    cond = np.array(g["cond"])
+    b = P.EOBBatch(add, max_step=g["max_step"], device=dev)
+    b.set_initial_state(cond)
+    print("plunge-edge binaries, fixed step", flush=True)
+    b.integrate(rtol=g["rtol"], atol=g["rtol"], hInit=g["hInit"], fix_step=True, group=64)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(b.num_steps.cpu().numpy(), g["num_steps"])
+    np.testing.assert_array_equal(b.stop_info.cpu().numpy(), g["stop_info"])
+    for i, s in enumerate(g["samples"]):
+        d = b.dense[i, torch.as_tensor(s["rows"], device=dev)].cpu().numpy()
+        np.testing.assert_allclose(d[:, 0], s["t"], rtol=1e-11, atol=1e-9)
+        np.testing.assert_allclose(d[:, 1:], np.array(s["state"]), rtol=3e-9, atol=1e-13)
+    b2 = P.EOBBatch(add, max_step=4096, device=dev)
+    b2.set_initial_state(cond)
+    b2.integrate(rtol=1e-10, atol=1e-10, group=64)
+    torch.cuda.synchronize()
+    si = b2.stop_info.cpu().numpy()
+    assert set(si.tolist()) <= {-1, 4} | set(g["stop_info"]), si
