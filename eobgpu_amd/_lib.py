@@ -98,6 +98,17 @@ _SIGS.update({
     "eobhip_fd_likelihood_static": [c_vp, c_vp, c_int, c_int, c_double, c_double, c_vp, c_vp],
 })
 _RESTYPES["eobhip_fd_likelihood_static_max_partials"] = c_size
+_SIGS.update({
+    "eobhip_imr_fit": [c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_int, c_vp, c_vp],
+    "eobhip_kerr_qnm": [c_int, c_int, c_vp, c_int, c_vp, c_vp],
+    "eobhip_imr_params": [c_vp] * 9 + [c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_imr_resample": [c_vp] * 6 + [c_int, c_int, c_int, c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "eobhip_imr_rd_doubles": [],
+    "eobhip_imr_bin_doubles": [],
+    "eobhip_imr_bin_ints": [],
+})
+for _k in ("eobhip_imr_rd_doubles", "eobhip_imr_bin_doubles", "eobhip_imr_bin_ints"):
+    _RESTYPES[_k] = c_size
 _RESTYPES["eobhip_fd_likelihood_partial_doubles"] = c_size
 _RESTYPES["eobhip_spline_scratch_doubles"] = c_size
 _RESTYPES["eobhip_spline_series_scratch_doubles"] = c_size

@@ -15,7 +15,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libeobhip.so")
-SOURCES = ["eob_kernels.hip", "spline_kernels.hip", "likelihood_kernels.hip", "detector_kernels.hip"]
+SOURCES = ["eob_kernels.hip", "spline_kernels.hip", "likelihood_kernels.hip", "detector_kernels.hip", "imr_kernels.hip"]
 ARCH = os.environ.get("EOBHIP_ARCH", "gfx950")
 
 FLAGS = [

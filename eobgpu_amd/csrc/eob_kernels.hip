@@ -1056,7 +1056,7 @@ int run_coupled(const IntegParams& P, PlainState S, CoupledState C, int* act0, i
 // Registers capped at three wavefronts per SIMD (168, a few spills): with the
 // 284 it would take, a wavefront could not share a SIMD with an integrating one
 // (292 of 512), so knots of finished binaries waited for whole free SIMDs.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
                                                         const ModeCoef* modes, int n, int max_step, double* knots) {
     const int b = blockIdx.y;
     if (b >= n) return;
@@ -1108,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void k_wave_tables_batch(const double* add,
 // The listed modes on knots, one thread per knot: get_hlms (eob/seobnrv4phm.py:2138-2221)
 // and the series compute_full_waveform splines (:2336-2356).  knots
 // [n][max_step][2 nm + 1] = Re(h_j e^{i m_j phi}) for j < nm, their Im, phi.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_hlm_modes_knots(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void k_hlm_modes_knots(
     const double* dense, const int* num_steps, const BinData* bdp, const WaveMode* wtab, const double* add, int n, int max_step, int nm,
     double* knots) {
     const int b = blockIdx.y;

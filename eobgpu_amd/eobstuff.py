@@ -1,0 +1,266 @@
+"""Drop-in for the reference's ``eob/eobstuff.py`` (remnant, QNM and ringdown
+fits) and for the NR peak fits and ringdown ansatz of ``eob/seobnrv4phm.py``
+that use them (:878-1462, :1686-1700, :1779-1844).
+
+Same function names and arguments.  Every fit is evaluated by one elementwise
+HIP kernel (``eobhip_imr_fit``), the QNM frequencies by Leaver's continued
+fractions on the device (``eobhip_kerr_qnm``); the few elementwise
+combinations around them run as torch operations on the GPU.  Inputs may be
+numpy arrays / scalars or torch tensors: numpy in gives numpy out, a CUDA
+tensor in gives a CUDA tensor out.
+
+Two functions replace libraries the reference imports and this image lacks:
+``bbh_final_spin_non_precessing_HBR2016`` (pesummary) is the published HBR2016
+fit, and ``compute_QNM`` (lalsimulation's tabulated QNMs) solves the Teukolsky
+equation by Leaver's method.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._lib import call, lib
+from .utils import current_stream
+
+__all__ = [
+    "compute_final_mass_SEOBNRv4", "bbh_final_spin_non_precessing_HBR2016", "compute_QNM",
+    "EOBCalculateRDAmplitudeCoefficient1", "EOBCalculateRDAmplitudeCoefficient2", "EOBCalculateRDPhaseCoefficient1",
+    "EOBCalculateRDPhaseCoefficient2", "EOBCalculateRDAmplitudeConstraintedCoefficient1",
+    "EOBCalculateRDAmplitudeConstraintedCoefficient2", "EOBCalculateRDPhaseConstraintedCoefficient1",
+    "EOBGetNRSpinPeakDeltaTv4", "EOBGetNRSpinPeakAmplitudeV4", "EOBGetNRSpinPeakADotV4", "EOBGetNRSpinPeakADDotV4",
+    "EOBGetNRSpinPeakOmegaV4", "EOBGetNRSpinPeakOmegaDotV4", "EOBNonQCCorrection", "compute_MR_mode_free",
+]
+
+# which-codes of eobhip_imr_fit (include/eobhip.h)
+_FIT = dict(DT22=0, AMP=1, ADOT=2, ADDOT=3, OMEGA=4, OMEGADOT=5, C1F=6, C2F=7, D1F=8, D2F=9, FINAL_MASS=10, FINAL_SPIN=11,
+            ISCO_RADIUS=12)
+_DEV = "cuda"
+
+
+def _dev(x, shape=None):
+    t = x.to(device=_DEV, dtype=torch.float64) if isinstance(x, torch.Tensor) else torch.as_tensor(
+        np.asarray(x, dtype=np.float64), device=_DEV)
+    if shape is not None:
+        t = t.expand(shape)
+    return t.contiguous()
+
+
+def _back(t, like):
+    if any(isinstance(x, torch.Tensor) for x in like):
+        return t
+    out = t.cpu().numpy()
+    return out if out.ndim else float(out)
+
+
+def _fit(which, l, m, *xs):
+    shape = torch.broadcast_shapes(*[torch.as_tensor(np.asarray(x)).shape if not isinstance(x, torch.Tensor) else x.shape
+                                     for x in xs])
+    ts = [_dev(x, shape) for x in xs] + [None] * (4 - len(xs))
+    out = torch.empty(shape, dtype=torch.float64, device=_DEV)
+    n = out.numel()
+    call("eobhip_imr_fit", _FIT[which], int(l), int(m), *[None if t is None else t.data_ptr() for t in ts], n, out.data_ptr(),
+         current_stream())
+    return _back(out, xs)
+
+
+def _per_mode(which, ell, m, eta, chi):
+    """[nbin][nmodes] for mode lists ell, m (the reference's layout, eob/eobstuff.py:108)."""
+    ell = np.atleast_1d(np.asarray(ell.cpu() if isinstance(ell, torch.Tensor) else ell)).astype(int)
+    m = np.atleast_1d(np.asarray(m.cpu() if isinstance(m, torch.Tensor) else m)).astype(int)
+    cols = [_fit(which, li, mi, eta, chi) for li, mi in zip(ell, m)]
+    if isinstance(cols[0], torch.Tensor):
+        return torch.stack(cols, dim=-1)
+    return np.stack([np.asarray(c) for c in cols], axis=-1)
+
+
+# ------------------------------------------------------------------ remnant
+def compute_final_mass_SEOBNRv4(mass1, mass2, a1, a2):
+    """eob/eobstuff.py:6-19: remnant mass in units of mass1 + mass2."""
+    return _fit("FINAL_MASS", 0, 0, mass1, mass2, a1, a2)
+
+
+def bbh_final_spin_non_precessing_HBR2016(m1, m2, chi1, chi2, version="M3J4"):
+    """pesummary.gw.conversions.nrutils.bbh_final_spin_non_precessing_HBR2016 as the
+    reference calls it (eob/seobnrv4phm.py:2459-2461, version "M3J4"): the HBR2016
+    fit with n_M = 3, n_J = 4 (other versions are not provided)."""
+    if version != "M3J4":
+        raise NotImplementedError("only the M3J4 version of the HBR2016 fit is provided")
+    return _fit("FINAL_SPIN", 0, 0, m1, m2, chi1, chi2)
+
+
+def compute_QNM(ell, m, n, af, Mf):
+    """eob/eobstuff.py:32-41: the (ell, m, 0) QNM frequency of the remnant in units of the
+    total mass, conj of lalsimulation's omega_r + i / tau, i.e. omega_r - i / tau."""
+    if int(n) != 0:
+        raise NotImplementedError("only the fundamental mode (n = 0) is provided")
+    like = (af, Mf)
+    shape = torch.broadcast_shapes(*[x.shape if isinstance(x, torch.Tensor) else np.shape(x) for x in like])
+    a = _dev(af, shape)
+    out = torch.empty(tuple(shape) + (2,), dtype=torch.float64, device=_DEV)
+    call("eobhip_kerr_qnm", int(ell), int(m), a.data_ptr(), a.numel(), out.data_ptr(), current_stream())
+    w = torch.complex(out[..., 0], out[..., 1]) / _dev(Mf, shape)
+    if any(isinstance(x, torch.Tensor) for x in like):
+        return w
+    w = w.cpu().numpy()
+    return w if w.ndim else complex(w)
+
+
+# ------------------------------------------------------------ ringdown fits
+def EOBCalculateRDAmplitudeCoefficient1(ell, m, eta, chi, xp=None):
+    """eob/eobstuff.py:44-108 -> [nbin][nmodes]."""
+    return _per_mode("C1F", ell, m, eta, chi)
+
+
+def EOBCalculateRDAmplitudeCoefficient2(ell, m, eta, chi, xp=None):
+    """eob/eobstuff.py:111-178."""
+    return _per_mode("C2F", ell, m, eta, chi)
+
+
+def EOBCalculateRDPhaseCoefficient1(ell, m, eta, chi, xp=None):
+    """eob/eobstuff.py:181-248."""
+    return _per_mode("D1F", ell, m, eta, chi)
+
+
+def EOBCalculateRDPhaseCoefficient2(ell, m, eta, chi, xp=None):
+    """eob/eobstuff.py:251-322."""
+    return _per_mode("D2F", ell, m, eta, chi)
+
+
+def _ew(*xs):
+    like = xs
+    ts = [x.to(device=_DEV, dtype=torch.float64) if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, np.float64), device=_DEV)
+          for x in xs]
+    return ts, like
+
+
+def EOBCalculateRDAmplitudeConstraintedCoefficient1(c1f, c2f, sigmaR, amp, damp, eta, xp=None):
+    """eob/eobstuff.py:325-329: amplitude continuity and slope at t_match."""
+    (c1f, c2f, sR, a, da, eta), like = _ew(c1f, c2f, sigmaR, amp, damp, eta)
+    return _back(1 / (c1f * eta) * (da - sR * a) * torch.cosh(c2f) ** 2, like)
+
+
+def EOBCalculateRDAmplitudeConstraintedCoefficient2(c1f, c2f, sigmaR, amp, damp, eta, xp=None):
+    """eob/eobstuff.py:332-335."""
+    (c1f, c2f, sR, a, da, eta), like = _ew(c1f, c2f, sigmaR, amp, damp, eta)
+    return _back(a / eta - 1 / (c1f * eta) * (da - sR * a) * torch.cosh(c2f) * torch.sinh(c2f), like)
+
+
+def EOBCalculateRDPhaseConstraintedCoefficient1(d1f, d2f, sigmaI, omega, xp=None):
+    """eob/eobstuff.py:338-342: frequency continuity at t_match."""
+    (d1f, d2f, sI, w), like = _ew(d1f, d2f, sigmaI, omega)
+    return _back((w - sI) * (1 + d2f) / (d1f * d2f), like)
+
+
+# ------------------------------------------------------------ NR peak fits
+def _vec(fn):
+    def per_mode(ell, m, *args):
+        ell_a, m_a = np.asarray(ell.cpu() if isinstance(ell, torch.Tensor) else ell), np.asarray(
+            m.cpu() if isinstance(m, torch.Tensor) else m)
+        if ell_a.ndim == 0:
+            return fn(int(ell_a), int(m_a), *args)
+        shape = np.broadcast_shapes(ell_a.shape, *[tuple(a.shape) if hasattr(a, "shape") else () for a in args])
+        el, mm = np.broadcast_to(ell_a, shape).ravel(), np.broadcast_to(m_a, shape).ravel()
+        flat = [np.broadcast_to(np.asarray(a.cpu() if isinstance(a, torch.Tensor) else a, np.float64), shape).ravel() for a in args]
+        out = np.empty(el.size)
+        for key in set(zip(el.tolist(), mm.tolist())):
+            sel = (el == key[0]) & (mm == key[1])
+            out[sel] = np.asarray(fn(key[0], key[1], *[f[sel] for f in flat]))
+        out = out.reshape(shape)
+        if any(isinstance(a, torch.Tensor) for a in args):
+            return torch.as_tensor(out, device=_DEV)
+        return out
+
+    per_mode.__name__ = fn.__name__
+    per_mode.__doc__ = fn.__doc__
+    return per_mode
+
+
+def EOBGetNRSpinPeakDeltaTv4(ell, m, m1, m2, chi1, chi2):
+    """eob/seobnrv4phm.py:878-929: the (2,2) peak offset, plus 10 M for (5,5)."""
+    out = _fit("DT22", 2, 2, m1, m2, chi1, chi2)
+    return out + 10.0 if (int(ell), int(m)) == (5, 5) else out
+
+
+@_vec
+def EOBGetNRSpinPeakAmplitudeV4(ell, m, m1, m2, chiS, chiA):
+    """eob/seobnrv4phm.py:945-1075."""
+    return _fit("AMP", ell, m, m1, m2, chiS, chiA)
+
+
+@_vec
+def EOBGetNRSpinPeakADotV4(ell, m, m1, m2, chiS, chiA):
+    """eob/seobnrv4phm.py:1078-1158."""
+    return _fit("ADOT", ell, m, m1, m2, chiS, chiA)
+
+
+@_vec
+def EOBGetNRSpinPeakADDotV4(ell, m, m1, m2, chiS, chiA):
+    """eob/seobnrv4phm.py:1162-1258."""
+    return _fit("ADDOT", ell, m, m1, m2, chiS, chiA)
+
+
+@_vec
+def EOBGetNRSpinPeakOmegaV4(ell, m, eta, a):
+    """eob/seobnrv4phm.py:1261-1359."""
+    return _fit("OMEGA", ell, m, eta, a)
+
+
+@_vec
+def EOBGetNRSpinPeakOmegaDotV4(ell, m, eta, a):
+    """eob/seobnrv4phm.py:1363-1462."""
+    return _fit("OMEGADOT", ell, m, eta, a)
+
+
+# ---------------------------------------------------- NQC factor, ringdown
+def EOBNonQCCorrection(r, phi, pr, pphi, omega, coeffs, xp=None):
+    """eob/seobnrv4phm.py:1686-1700: the complex NQC factor from the coefficients
+    dict (a1 a2 a3 a3S a4 a5 b1 b2 b3 b4)."""
+    (r, pr, om), like = _ew(r, pr, omega)
+    c = {k: (v.to(device=_DEV, dtype=torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, np.float64),
+                                                                                                         device=_DEV))
+         for k, v in coeffs.items()}
+    sqrtR = torch.sqrt(r)
+    rO = r * om
+    mag = 1.0 + (pr * pr / (rO * rO)) * (c["a1"] + c["a2"] / r + (c["a3"] + c["a3S"]) / (r * sqrtR) + c["a4"] / (r * r)
+                                         + c["a5"] / (r * r * sqrtR))
+    ph = c["b1"] * pr / rO + pr * pr * pr / rO * (c["b2"] + c["b3"] / sqrtR + c["b4"] / r)
+    out = torch.complex(mag * torch.cos(ph), mag * torch.sin(ph))
+    return out if any(isinstance(x, torch.Tensor) for x in like) else out.cpu().numpy()
+
+
+def compute_MR_mode_free(t, m1, m2, chi1, chi2, attach_params, ell, m, t_match=0, phi_match=0, debug=False, xp=None):
+    """eob/seobnrv4phm.py:1779-1844: the ringdown modes h [nbin][nmodes][len(t)] and their
+    phase, from the attachment values (amp, damp, omega [nbin][nmodes]) and the remnant
+    (final_mass, final_spin [nbin])."""
+    (t, m1, m2, chi1, chi2), like = _ew(t, m1, m2, chi1, chi2)
+    ap = {k: (v.to(device=_DEV, dtype=torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, np.float64),
+                                                                                                         device=_DEV))
+          for k, v in attach_params.items()}
+    t_match = _dev(t_match)
+    phi_match = _dev(phi_match)
+    eta = m1 * m2 / (m1 + m2) ** 2
+    chi = 0.5 * (chi1 + chi2) + 0.5 * (chi1 - chi2) * (m1 - m2) / (m1 + m2) / (1.0 - 2.0 * eta)
+    ell_a = np.atleast_1d(np.asarray(ell.cpu() if isinstance(ell, torch.Tensor) else ell)).astype(int)
+    m_a = np.atleast_1d(np.asarray(m.cpu() if isinstance(m, torch.Tensor) else m)).astype(int)
+    w = torch.stack([compute_QNM(int(li), int(mi), 0, ap["final_spin"], ap["final_mass"]).conj()
+                     for li, mi in zip(ell_a, m_a)], dim=-1)   # lalsimulation's convention
+    sigmaR, sigmaI = -w.imag, -w.real
+    c1f = EOBCalculateRDAmplitudeCoefficient1(ell_a, m_a, eta, chi)
+    c2f = EOBCalculateRDAmplitudeCoefficient2(ell_a, m_a, eta, chi)
+    fix = torch.as_tensor((ell_a == 2) & (m_a == 2), device=_DEV)[None, :] & (sigmaR > 2.0 * c1f * torch.tanh(c2f))
+    c1f = torch.where(fix, sigmaR / (2.0 * torch.tanh(c2f)), c1f)
+    d1f = EOBCalculateRDPhaseCoefficient1(ell_a, m_a, eta, chi)
+    d2f = EOBCalculateRDPhaseCoefficient2(ell_a, m_a, eta, chi)
+    e = eta[:, None]
+    c1c = EOBCalculateRDAmplitudeConstraintedCoefficient1(c1f, c2f, sigmaR, ap["amp"], ap["damp"], e)
+    c2c = EOBCalculateRDAmplitudeConstraintedCoefficient2(c1f, c2f, sigmaR, ap["amp"], ap["damp"], e)
+    d1c = EOBCalculateRDPhaseConstraintedCoefficient1(d1f, d2f, sigmaI, ap["omega"])
+    x = t[:, None, :] - t_match.reshape(-1, 1, 1)
+    Alm = c1c[:, :, None] * torch.tanh(c1f[:, :, None] * x + c2f[:, :, None]) + c2c[:, :, None]
+    philm = phi_match.reshape(t.shape[0], -1)[:, :, None] - d1c[:, :, None] * torch.log(
+        (1 + d2f[:, :, None] * torch.exp(-d1f[:, :, None] * x)) / (1 + d2f[:, :, None]))
+    tw = torch.complex(-w.real, w.imag)
+    hlm = eta[:, None, None] * Alm * torch.exp(1j * philm) * torch.exp(1j * tw[:, :, None] * x)
+    if any(isinstance(v, torch.Tensor) for v in like):
+        return hlm, philm
+    return hlm.cpu().numpy(), philm.cpu().numpy()

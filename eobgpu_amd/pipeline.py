@@ -356,6 +356,67 @@ class EOBBatch:
         return hp, hc
 
 
+    # ------------------------------------------------- merger-ringdown attachment
+    def build_rp_splines(self):
+        """Not-a-knot splines of r and pr over the knots, rp_coef [n][max_step][8]
+        (compute_full_waveform's r_pr_spline, eob/seobnrv4phm.py:2327-2334)."""
+        knots = self.dense[:, :, [1, 3]].contiguous()
+        self.rp_coef = torch.empty((self.n, self.max_step, 8), dtype=torch.float64, device=self.device)
+        scratch = torch.empty(int(lib().eobhip_spline_series_scratch_doubles(self.n, self.max_step, 2)), dtype=torch.float64,
+                              device=self.device)
+        call("eobhip_spline_build_series", self.dense.data_ptr(), knots.data_ptr(), self.num_steps.data_ptr(), self.n, self.max_step,
+             2, self.rp_coef.data_ptr(), scratch.data_ptr(), current_stream())
+        self._keep_rp = (knots, scratch)
+        return self.rp_coef
+
+    def imr_params(self, total_mass_msun, dt):
+        """NQC coefficients, remnant, QNMs and ringdown coefficients of every binary
+        and mode of ``hlm_modes_knots`` (which must hold only modes with ringdown
+        fits: (2,2), (2,1), (3,3), (4,4), (5,5)), for the grid t_j = t_0 + j dt
+        seconds (``eobhip_imr_params``).  Needs ``build_mode_splines`` first; builds
+        the r, pr splines.  Returns a dict of device tensors."""
+        ls, ms = (list(x) for x in self._wave_key)
+        nm = len(ls)
+        L = lib()
+        self.build_rp_splines()
+        M = torch.as_tensor(np.broadcast_to(np.asarray(total_mass_msun, dtype=np.float64), (self.n,)).copy(), device=self.device)
+        dT_M = (dt / (M * MTSUN_SI)).contiguous()
+        dev = dict(dtype=torch.float64, device=self.device)
+        out = dict(qnm=torch.empty((self.n, nm, 2), **dev), nqc=torch.empty((self.n, nm, 5), **dev),
+                   rd=torch.empty((self.n, nm, int(L.eobhip_imr_rd_doubles())), **dev),
+                   binf=torch.zeros((self.n, int(L.eobhip_imr_bin_doubles())), **dev),
+                   bini=torch.zeros((self.n, int(L.eobhip_imr_bin_ints())), dtype=torch.int32, device=self.device), dT_M=dT_M)
+        call("eobhip_imr_params", self.add[0].data_ptr(), self.add[1].data_ptr(), self.add[2].data_ptr(), self.add[3].data_ptr(),
+             self.dense.data_ptr(), self.num_steps.data_ptr(), self.mode_coef.data_ptr(), self.rp_coef.data_ptr(), dT_M.data_ptr(), self.n,
+             self.max_step, nm, _host_ints(ls), _host_ints(ms), out["qnm"].data_ptr(), out["nqc"].data_ptr(), out["rd"].data_ptr(),
+             out["binf"].data_ptr(), out["bini"].data_ptr(), current_stream())
+        self.imr = out
+        return out
+
+    def imr_template(self, total_mass_msun, dt, N, amp=None, batch_ringdown=False):
+        """(h+, hx) [n][N]: the IMR template -- amp times the sum over the modes of
+        the NQC-corrected modes up to the attachment sample and of the ringdown
+        after it (compute_full_waveform, eob/seobnrv4phm.py:2308-2556, summed as
+        BBHWaveformTD does, :720) on t_j = t_0 + j dt seconds, j < N.  Each binary's
+        ringdown lasts its own 30 (2,2) damping times; batch_ringdown=True pads every
+        binary to the batch's longest, as the reference does (:2514)."""
+        p = self.imr_params(total_mass_msun, dt)
+        if batch_ringdown:
+            p["bini"][:, 2] = p["bini"][:, 2].max()
+        ls, ms = (list(x) for x in self._wave_key)
+        a = torch.as_tensor(np.broadcast_to(np.asarray(1.0 if amp is None else amp, dtype=np.float64), (self.n,)).copy(),
+                            device=self.device)
+        hp = torch.empty((self.n, N), dtype=torch.float64, device=self.device)
+        hc = torch.empty_like(hp)
+        self.nsamp = torch.zeros(self.n, dtype=torch.int32, device=self.device)
+        call("eobhip_imr_resample", self.dense.data_ptr(), self.num_steps.data_ptr(), self.mode_coef.data_ptr(), self.rp_coef.data_ptr(),
+             a.data_ptr(), p["dT_M"].data_ptr(), self.n, self.max_step, int(N), len(ls), _host_ints(ls), _host_ints(ms),
+             p["nqc"].data_ptr(), p["rd"].data_ptr(), p["binf"].data_ptr(), p["bini"].data_ptr(), hp.data_ptr(), hc.data_ptr(),
+             self.nsamp.data_ptr(), current_stream())
+        self._keep = (a,)
+        return hp, hc
+
+
 def _mode_arrays(modes):
     modes = [(int(l), int(m)) for l, m in modes]
     if not 1 <= len(modes) <= 15:

@@ -364,6 +364,65 @@ int eobhip_lik_finish(const double* X, const long long* xoff, const int* mlog2, 
                       const double* bank, const double* cross_partial, double* hh_partial, const double* dd, double df, double* logl,
                       double* terms, void* stream);
 
+/* ---------------------------------------------------- merger-ringdown attachment
+ * The fits of the NQC and ringdown attachment, evaluated elementwise on device
+ * arrays of n: which selects the function (EOBHIP_FIT_*), (l, m) the mode, x1..x4
+ * its arguments in the reference's order (unused ones may be null):
+ *   DT22       EOBGetNRSpinPeakDeltaTv4(2, 2, m1, m2, chi1, chi2)  eob/seobnrv4phm.py:878
+ *   AMP/ADOT/ADDOT  EOBGetNRSpinPeak{Amplitude,ADot,ADDot}V4(l, m, m1, m2, chiS, chiA)  :945, :1078, :1162
+ *   OMEGA/OMEGADOT  EOBGetNRSpinPeak{Omega,OmegaDot}V4(l, m, eta, a)  :1261, :1363
+ *   C1F/C2F/D1F/D2F EOBCalculateRD{Amplitude,Phase}Coefficient{1,2}(l, m, eta, chi)  eob/eobstuff.py:44-322
+ *   FINAL_MASS compute_final_mass_SEOBNRv4(m1, m2, a1, a2)  eob/eobstuff.py:6
+ *   FINAL_SPIN bbh_final_spin_non_precessing_HBR2016(m1, m2, chi1, chi2, "M3J4")  (pesummary; called at
+ *              eob/seobnrv4phm.py:2459): the published HBR2016 fit
+ *   ISCO_RADIUS the Kerr ISCO radius of spin x1 (lalsimulation's SimRadiusKerrISCO, eob/eobstuff.py:15) */
+#define EOBHIP_FIT_DT22 0
+#define EOBHIP_FIT_AMP 1
+#define EOBHIP_FIT_ADOT 2
+#define EOBHIP_FIT_ADDOT 3
+#define EOBHIP_FIT_OMEGA 4
+#define EOBHIP_FIT_OMEGADOT 5
+#define EOBHIP_FIT_C1F 6
+#define EOBHIP_FIT_C2F 7
+#define EOBHIP_FIT_D1F 8
+#define EOBHIP_FIT_D2F 9
+#define EOBHIP_FIT_FINAL_MASS 10
+#define EOBHIP_FIT_FINAL_SPIN 11
+#define EOBHIP_FIT_ISCO_RADIUS 12
+#define EOBHIP_FIT_COUNT 13
+int eobhip_imr_fit(int which, int l, int m, const double* x1, const double* x2, const double* x3, const double* x4, int n, double* out,
+                   void* stream);
+/* Kerr QNM frequency M omega_lm0 (e^{-i omega t}, Im < 0; omega complex [n]) of holes of spin af[n]:
+ * Leaver's continued fractions, replacing lalsimulation's SimIMREOBGenerateQNMFreqV2fromFinal
+ * behind compute_QNM (eob/eobstuff.py:32-41, which returns conj(omega) / M_f). 2 <= l <= 5. */
+int eobhip_kerr_qnm(int l, int m, const double* af, int n, double* omega, void* stream);
+/* The attachment's parameters for n binaries and nm <= 8 modes (ls, ms host arrays; each
+ * mode one of (2,2), (2,1), (3,3), (4,4), (5,5)), compute_full_waveform
+ * (eob/seobnrv4phm.py:2308-2519) per binary: masses and spins m1, m2, chi1, chi2 [n] (any
+ * mass unit), the trajectory (dense, num_steps), the mode splines coef [n][max_step][4 (2 nm + 1)]
+ * (eobhip_spline_build_series over eobhip_hlm_modes_knots' series) and the splines of r and pr
+ * rp_coef [n][max_step][8], grid spacing dT_M [n] (units of M).  Outputs: qnm [n][nm] complex
+ * (lalsimulation's convention omega_r + i / tau, units of M), nqc [n][nm][5] = a1 a2 a3 b1 b2
+ * (EOBCalculateNQCCoefficientsV4_freeattach, :1465-1683), rd [n][nm][eobhip_imr_rd_doubles()]
+ * (attach values amp, damp, phi, omega; c1c c2c c1f c2f d1c d1f d2f; omega_r, 1/tau; eta),
+ * binf [n][eobhip_imr_bin_doubles()] (t_peak, t_match, dt, M_f, a_f, DeltaT22), bini
+ * [n][eobhip_imr_bin_ints()] (grid samples, attachment sample, ringdown samples: this
+ * binary's own 30 damping times, where the reference pads every binary to the batch's longest). */
+int eobhip_imr_params(const double* m1, const double* m2, const double* chi1, const double* chi2, const double* dense,
+                      const int* num_steps, const double* coef, const double* rp_coef, const double* dT_M, int n, int max_step, int nm,
+                      const int* ls, const int* ms, double* qnm, double* nqc, double* rd, double* binf, int* bini, void* stream);
+/* The IMR template on t_j = t_0 + j dT (j < N): amp [n] times the sum over modes of the
+ * NQC-corrected modes (EOBNonQCCorrection, :1686-1700) up to the attachment sample and of the
+ * ringdown (compute_MR_mode_free, :1779-1844) for the ringdown samples after it, 0 beyond;
+ * hplus = Re, hcross = Im [n][N] (BBHWaveformTD's template_channels, :720); nsamp [n] the
+ * nonzero length.  The parameters are eobhip_imr_params' outputs. */
+int eobhip_imr_resample(const double* dense, const int* num_steps, const double* coef, const double* rp_coef, const double* amp,
+                        const double* dT_M, int n, int max_step, int N, int nm, const int* ls, const int* ms, const double* nqc,
+                        const double* rd, const double* binf, const int* bini, double* hplus, double* hcross, int* nsamp, void* stream);
+size_t eobhip_imr_rd_doubles(void);
+size_t eobhip_imr_bin_doubles(void);
+size_t eobhip_imr_bin_ints(void);
+
 #ifdef __cplusplus
 }
 #endif
