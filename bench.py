@@ -74,9 +74,10 @@ def parse():
                     help="skip timing the pass captured as one HIP graph (pipeline.GraphPass, full-length likelihood)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
-    ap.add_argument("--workload", choices=("uneven", "spline", "capacity"), default="uneven",
+    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr"), default="uneven",
                     help="uneven: the headline 1024-binary pipeline; spline: 512 synthetic (2,2) modes with 5k-60k knots "
-                         "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator")
+                         "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator; "
+                         "imr: the headline binaries' IMR templates (five modes, NQC and ringdown) at 16384 Hz")
     return ap.parse_args()
 
 
@@ -520,6 +521,59 @@ def workload_capacity(args):
     print(json.dumps(out))
 
 
+def workload_imr(args):
+    """The waveform generator of the headline batch: for each of the 1024 uneven
+    binaries, fresh parameters ingested on the device, initial conditions, the
+    two-pass trajectory, the five modes on the knots, their splines, the NQC and
+    ringdown parameters and the IMR template summed over the modes on 2^20
+    samples at 1/16384 s (SEOBNRv4PHM.__call__ with compute_full_waveform and
+    BBHWaveformTD's mode sum, eob/seobnrv4phm.py:2628-2726, 2275-2556, 720)."""
+    import torch
+    from eobgpu_amd import configs, pipeline as P
+    n = args.n
+    N = 1 << args.log2N
+    dt = 1.0 / 16384
+    ps = [configs.uneven_batch(n=n, seed=2 + k) for k in range(args.warmup + args.steps)]
+    b = P.EOBBatch.from_params(ps[0]["m1"], ps[0]["m2"], ps[0]["chi1"], ps[0]["chi2"], max_step=args.max_step)
+    names = ("setup_ic", "trajectory", "knots_splines", "imr_params", "imr_template")
+    times, stages = [], []
+    for it, p in enumerate(ps):
+        M = p["m1"] + p["m2"]
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        ev[0].record()
+        b.set_params(p["m1"], p["m2"], p["chi1"], p["chi2"])
+        b.initial_conditions(p["r0"])
+        ev[1].record()
+        b.integrate(rtol=args.rtol, atol=args.rtol, two_pass=True)
+        ev[2].record()
+        b.hlm_modes_knots(P.REFERENCE_MODES)
+        b.build_mode_splines()
+        ev[3].record()
+        b.imr_params(M, dt)
+        ev[4].record()
+        hp, hc = b.imr_template(M, dt, N)
+        ev[5].record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(ev[0].elapsed_time(ev[-1]))
+            stages.append([ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))])
+        del hp, hc
+    ms = float(np.mean(times))
+    st = np.mean(np.array(stages), axis=0)
+    bini = b.imr["bini"].cpu().numpy()
+    nqc = b.imr["nqc"].cpu().numpy()
+    ok = np.isfinite(nqc).all(axis=(1, 2)) & (bini[:, 1] >= 0)
+    out = {"metric": "IMR waveforms/s (1024 uneven binaries, five modes with NQC and ringdown)", "value": n / (ms * 1e-3),
+           "unit": "waveforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[15,40] M, fresh "
+                                  "parameters each step; modes (2,2),(2,1),(3,3),(4,4),(5,5); NQC + ringdown; 2^%d samples at "
+                                  "1/16384 s" % args.log2N, "global_batch": n, "seq_len": N},
+           "stages_ms": dict(zip(names, [float(x) for x in st])),
+           "attached_binaries": int(ok.sum()), "ringdown_samples": {"min": int(bini[ok, 2].min()), "max": int(bini[ok, 2].max())}}
+    print(json.dumps(out))
+
+
 def capacity_served(p, args, budget_gb=6.0, N=1 << 12, dt=1.0 / 2048):
     """The same 16384 binaries as 64 requests of 256 arriving at once, served
     under one memory budget in two ways: admission by projected footprint (size
@@ -582,6 +636,8 @@ def main():
         return workload_spline(args)
     if args.workload == "capacity":
         return workload_capacity(args)
+    if args.workload == "imr":
+        return workload_imr(args)
     pipe = Pipeline(args.n, args.log2N, args.max_step, args.rtol)
     pipe.make_data()
     print(f"[bench] data ready, n={args.n} N=2^{args.log2N}", file=sys.stderr, flush=True)

@@ -1056,7 +1056,7 @@ int run_coupled(const IntegParams& P, PlainState S, CoupledState C, int* act0, i
 // Registers capped at three wavefronts per SIMD (168, a few spills): with the
 // 284 it would take, a wavefront could not share a SIMD with an integrating one
 // (292 of 512), so knots of finished binaries waited for whole free SIMDs.
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(1))) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_hlm22_knots(const double* dense, const int* num_steps, const BinData* bdp,
                                                         const ModeCoef* modes, int n, int max_step, double* knots) {
     const int b = blockIdx.y;
     if (b >= n) return;

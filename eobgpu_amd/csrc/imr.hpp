@@ -419,7 +419,7 @@ IMR_HD double final_spin_hbr2016(double m1, double m2, double chi1, double chi2)
 // e^{-i omega t}).  The angular separation constant A solves the angular
 // continued fraction for c = a omega; omega solves the radial one.
 constexpr int kLeaverAngDepth = 60;
-constexpr int kLeaverRadDepth = 600;
+constexpr int kLeaverRadDepth = 300;   // n = 0 converged to 5e-15 up to a = 0.98
 
 // angular continued fraction beta_0 - alpha_0 gamma_1 / (beta_1 - ...)
 IMR_HD Cx leaver_angular_cf(Cx A, Cx c, int s, int m) {
@@ -478,44 +478,60 @@ IMR_HD Cx leaver_radial_cf(Cx w, double a, int s, int m, Cx* A) {
     return t;
 }
 
-// M omega_lm0 (M = 1, e^{-i omega t}: Im < 0) of a Kerr hole of spin af, by
-// continuation in the spin from the Schwarzschild root.
-IMR_HD Cx kerr_qnm(int l, int m, double af) {
+// The root of the radial continued fraction near w (Leaver units) at spin a,
+// with the separation constant followed from A: secant steps from w.
+struct QnmRoot {
+    Cx w, A;
+};
+IMR_HD QnmRoot leaver_refine(int m, double a, Cx w, Cx A, double dw) {
     const int s = -2;
-    if (!(fabs(af) <= 0.9999) || l < 2 || l > 5) return cx(NAN, NAN);
+    Cx x0 = w, x1 = w + cx(dw, -dw);
+    Cx A0 = A, A1 = A;
+    Cx f0 = leaver_radial_cf(x0, a, s, m, &A0);
+    Cx f1 = leaver_radial_cf(x1, a, s, m, &A1);
+    for (int it = 0; it < 50; ++it) {
+        const Cx d = f1 - f0;
+        if (cabs2(d) == 0.0) break;
+        const Cx x2 = x1 - f1 * (x1 - x0) / d;
+        x0 = x1;
+        f0 = f1;
+        A0 = A1;
+        x1 = x2;
+        f1 = leaver_radial_cf(x1, a, s, m, &A1);
+        if (cabs2(x1 - x0) < 1e-30 * (1.0 + cabs2(x1))) break;
+    }
+    return QnmRoot{x1, A1};
+}
+
+// The fundamental (l, m) root at spin af (units of M, |af| <= 0.9999) followed
+// from the Schwarzschild root in spin steps of 0.04; Leaver units out.  Only
+// the lowest l of each m (l = max(2, |m|)) is followed reliably: for higher l
+// the angular solve can settle on a lower l's separation constant.
+IMR_HD QnmRoot kerr_qnm_path(int l, int m, double af) {
+    const int s = -2;
     // Schwarzschild fundamental modes, l = 2..5, as the start (M omega)
     const Cx w0 = (l == 2) ? cx(0.3736716844180, -0.0889623156889)
                 : (l == 3) ? cx(0.5994432884375, -0.0927030479449)
                 : (l == 4) ? cx(0.8091783775322, -0.0941639609889)
                            : cx(1.0122953121354, -0.0948705160816);
-    Cx w = 2.0 * w0;   // Leaver units
-    Cx A = cx((double)((l - s) * (l + s + 1)));
-    const double a_end = 0.5 * af;                     // Leaver units
+    QnmRoot r{2.0 * w0, cx((double)((l - s) * (l + s + 1)))};   // Leaver units
+    const double a_end = 0.5 * af;
     const int nstep = 1 + (int)(fabs(a_end) / 0.02);
-    Cx wprev = w;
+    Cx wprev = r.w;
     for (int k = 1; k <= nstep; ++k) {
         const double a = a_end * (double)k / nstep;
-        Cx guess = (k > 1) ? (2.0 * w - wprev) : w;    // linear extrapolation along the path
-        Cx x0 = guess, x1 = guess + cx(1e-4, -1e-4);
-        Cx A0 = A, A1 = A;
-        Cx f0 = leaver_radial_cf(x0, a, s, m, &A0);
-        Cx f1 = leaver_radial_cf(x1, a, s, m, &A1);
-        for (int it = 0; it < 50; ++it) {
-            const Cx d = f1 - f0;
-            if (cabs2(d) == 0.0) break;
-            const Cx x2 = x1 - f1 * (x1 - x0) / d;
-            x0 = x1;
-            f0 = f1;
-            A0 = A1;
-            x1 = x2;
-            f1 = leaver_radial_cf(x1, a, s, m, &A1);
-            if (cabs2(x1 - x0) < 1e-30 * (1.0 + cabs2(x1))) break;
-        }
-        wprev = w;
-        w = x1;
-        A = A1;
+        const Cx guess = (k > 1) ? (2.0 * r.w - wprev) : r.w;   // linear extrapolation along the path
+        wprev = r.w;
+        r = leaver_refine(m, a, guess, r.A, 1e-4);
     }
-    return cx(0.5 * w.re, 0.5 * w.im);
+    return r;
+}
+
+// M omega_lm0 (M = 1, e^{-i omega t}: Im < 0) of a Kerr hole of spin af.
+IMR_HD Cx kerr_qnm(int l, int m, double af) {
+    if (!(fabs(af) <= 0.9999) || l < 2 || l > 5) return cx(NAN, NAN);
+    const QnmRoot r = kerr_qnm_path(l, m, af);
+    return cx(0.5 * r.w.re, 0.5 * r.w.im);
 }
 
 }  // namespace imr

@@ -19,6 +19,8 @@
 // wavefront samples a whole window, one sample per lane, in one pass.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "eobhip_common.hpp"
 #include "imr.hpp"
 #include "knot_window.hpp"
@@ -35,6 +37,7 @@ struct ImrModes {
     int l[kImrMaxModes];
     int m[kImrMaxModes];
 };
+constexpr int kImrMaxM = 5;     // the ringdown modes' largest m
 constexpr int kImrAfter = 31;   // window rows kept past the evaluation interval
 constexpr int kImrNqcBefore = 7; // rows before the NQC time (the reference's N = 7)
 constexpr int kImrSeries = 7;   // q1 A, q2 A, q3 A, A, p1, p2, phase
@@ -201,6 +204,70 @@ struct BinInputs {
     const double *m1, *m2, *chi1, *chi2;
 };
 
+// ------------------------------------------------------------- QNM table
+// The five ringdown modes' roots (omega and the separation constant, Leaver
+// units) on the spins -0.99 .. 0.99 in steps of 0.01, filled once per device;
+// a binary's QNM is then a few secant steps from the interpolated entry
+// instead of a walk from a = 0.  The mode index is imr::rd_mode_index.
+constexpr int kQnmGrid = 199;
+constexpr double kQnmA0 = -0.99, kQnmDa = 0.01;
+__device__ double g_qnm_tab[5][kQnmGrid][4];
+
+__device__ __forceinline__ void rd_mode_lm(int k, int& l, int& m) {
+    l = (k <= 1) ? 2 : k + 1;
+    m = (k == 1) ? 1 : l;
+}
+
+__global__ void k_qnm_table() {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 5 * kQnmGrid) return;
+    const int k = t / kQnmGrid, i = t % kQnmGrid;
+    int l, m;
+    rd_mode_lm(k, l, m);
+    const imr::QnmRoot r = imr::kerr_qnm_path(l, m, kQnmA0 + kQnmDa * i);
+    double* o = g_qnm_tab[k][i];
+    o[0] = r.w.re;
+    o[1] = r.w.im;
+    o[2] = r.A.re;
+    o[3] = r.A.im;
+}
+
+// M omega_lm0 at spin af: from the table for the ringdown modes, |af| <= 0.99;
+// otherwise along the spin path from a = 0
+__device__ Cx kerr_qnm_fast(int l, int m, double af) {
+    const int k = imr::rd_mode_index(l, m);
+    if (k < 0 || !(fabs(af) <= 0.99)) return imr::kerr_qnm(l, m, af);
+    const double x = (af - kQnmA0) / kQnmDa;
+    int i = (int)floor(x);
+    i = i < 0 ? 0 : (i > kQnmGrid - 2 ? kQnmGrid - 2 : i);
+    const double f = x - i;
+    const double* p0 = g_qnm_tab[k][i];
+    const double* p1 = g_qnm_tab[k][i + 1];
+    const Cx w = imr::cx((1.0 - f) * p0[0] + f * p1[0], (1.0 - f) * p0[1] + f * p1[1]);
+    const Cx A = imr::cx((1.0 - f) * p0[2] + f * p1[2], (1.0 - f) * p0[3] + f * p1[3]);
+    const imr::QnmRoot r = imr::leaver_refine(m, 0.5 * af, w, A, 1e-6);
+    return imr::cx(0.5 * r.w.re, 0.5 * r.w.im);
+}
+
+// fill the table on the current device once (stream-ordered, then waited for,
+// so that launches on any stream see it)
+std::mutex g_qnm_mu;
+bool g_qnm_ready[64];
+int ensure_qnm_table(void* stream) {
+    int dev = 0;
+    int rc = hip_status(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_qnm_mu);
+    if (dev >= 0 && dev < 64 && g_qnm_ready[dev]) return EOBHIP_OK;
+    hipLaunchKernelGGL(k_qnm_table, dim3(blocks_for(5 * kQnmGrid, 64)), dim3(64), 0, as_stream(stream));
+    rc = check_launch("qnm_table");
+    if (rc) return rc;
+    rc = hip_status(hipStreamSynchronize(as_stream(stream)), "qnm_table sync");
+    if (rc) return rc;
+    if (dev >= 0 && dev < 64) g_qnm_ready[dev] = true;
+    return EOBHIP_OK;
+}
+
 // ------------------------------------------------------------ QNMs per binary
 // Thread (b, j): j < nm the QNM of mode j, j == nm the (2,2) QNM that sets the
 // ringdown length (the reference's 30 damping times, :2463-2465 and :2514).
@@ -217,7 +284,7 @@ __global__ void k_imr_qnm(BinInputs in, const double* __restrict__ dense, const 
     const double mf = imr::final_mass_seobnrv4(m1 / M, m2 / M, c1, c2);
     const double af = imr::final_spin_hbr2016(m1 / M, m2 / M, c1, c2);
     const int l = (j < mm.nm) ? mm.l[j] : 2, m = (j < mm.nm) ? mm.m[j] : 2;
-    const Cx w = imr::kerr_qnm(l, m, af);
+    const Cx w = kerr_qnm_fast(l, m, af);
     const double wr = w.re / mf, wi = -w.im / mf;   // conj(omega) / M_f
     if (j < mm.nm) {
         wl[2 * ((size_t)b * mm.nm + j)] = wr;
@@ -435,6 +502,21 @@ __global__ __launch_bounds__(64) void k_imr_params(BinInputs in, const double* _
     }
 }
 
+// sin and cos of x: Taylor series through x^17 / x^18 for |x| < 1/2 (remainder
+// below 2e-23), the library's sincos otherwise.  The NQC phase of the
+// inspiral samples is small except near the attachment.
+__device__ __forceinline__ void sincos_nqc(double x, double* s, double* c) {
+    if (fabs(x) < 0.5) {
+        const double x2 = x * x;
+        *s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 + x2 * (-1.0 / 39916800
+                 + x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0 + x2 * (1.0 / 355687428096000.0)))))))));
+        *c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 + x2 * (1.0 / 479001600.0
+                 + x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0 + x2 * (-1.0 / 6402373705728000.0)))))))));
+    } else {
+        sincos(x, s, c);
+    }
+}
+
 // ------------------------------------------------------- the IMR template
 // h_j = amp (sum over modes of h_lm NQC_lm (t_j) for j <= idx, of the ringdown
 // for idx < j < idx + nadd, 0 after), t_j = t_0 + j dT; hplus = Re, hcross = Im.
@@ -492,15 +574,28 @@ __global__ __launch_bounds__(kImrBlock) void k_imr_resample(const double* __rest
                 const double P = pr * pr / (rO * rO);
                 const double ir = 1.0 / r, irs = 1.0 / (r * sr);
                 const double p1 = pr / rO, p3 = pr * pr * pr / rO;
+                // e^{-i k phi}, k = 1 .. 8, as powers of e^{-i phi}
+                double sp, cph;
+                sincos(phi, &sp, &cph);
+                double er[kImrMaxM + 1], ei[kImrMaxM + 1];
+                er[0] = 1.0; ei[0] = 0.0;
+#pragma unroll
+                for (int k = 1; k <= kImrMaxM; ++k) {
+                    er[k] = er[k - 1] * cph + ei[k - 1] * sp;
+                    ei[k] = ei[k - 1] * cph - er[k - 1] * sp;
+                }
                 for (int q = 0; q < nm; ++q) {
                     const double re = p.v(row + 4 * q), im = p.v(row + 4 * (nm + q));
-                    double s, c;
-                    sincos((double)mm.m[q] * phi, &s, &c);
-                    const double h_r = re * c + im * s, h_i = im * c - re * s;
+                    const int km = mm.m[q];
+                    double pr_ = 1.0, pi_ = 0.0;
+#pragma unroll
+                    for (int kk = 1; kk <= kImrMaxM; ++kk)
+                        if (kk == km) { pr_ = er[kk]; pi_ = ei[kk]; }
+                    const double h_r = re * pr_ - im * pi_, h_i = re * pi_ + im * pr_;
                     const double* nq = nqc + ((size_t)b * nm + q) * 5;
                     const double mag = 1.0 + P * (nq[0] + nq[1] * ir + nq[2] * irs);
                     double sn, cn;
-                    sincos(nq[3] * p1 + p3 * nq[4], &sn, &cn);
+                    sincos_nqc(nq[3] * p1 + p3 * nq[4], &sn, &cn);
                     const double fr = mag * cn, fi = mag * sn;
                     hr += h_r * fr - h_i * fi;
                     hi += h_r * fi + h_i * fr;
@@ -556,7 +651,7 @@ __global__ void k_imr_fit(int which, int l, int m, const double* __restrict__ x1
 __global__ void k_kerr_qnm(int l, int m, const double* __restrict__ af, int n, double* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const Cx w = imr::kerr_qnm(l, m, af[i]);
+    const Cx w = kerr_qnm_fast(l, m, af[i]);
     out[2 * i] = w.re;
     out[2 * i + 1] = w.im;
 }
@@ -585,8 +680,11 @@ int eobhip_imr_fit(int which, int l, int m, const double* x1, const double* x2, 
 }
 
 int eobhip_kerr_qnm(int l, int m, const double* af, int n, double* omega, void* stream) {
-    EOBHIP_REQUIRE(n >= 0 && l >= 2 && l <= 5 && m >= -l && m <= l, "bad arguments (2 <= l <= 5, |m| <= l)");
+    EOBHIP_REQUIRE(n >= 0 && l >= 2 && l <= 5 && m >= -l && m <= l && l == (abs(m) > 2 ? abs(m) : 2),
+                   "bad arguments (the fundamental mode of l = max(2, |m|), l <= 5)");
     if (n == 0) return EOBHIP_OK;
+    const int rc = ensure_qnm_table(stream);
+    if (rc) return rc;
     hipLaunchKernelGGL(k_kerr_qnm, dim3(blocks_for(n, 64)), dim3(64), 0, as_stream(stream), l, m, af, n, omega);
     return check_launch("kerr_qnm");
 }
@@ -599,6 +697,8 @@ int eobhip_imr_params(const double* m1, const double* m2, const double* chi1, co
     if (rc) return rc;
     EOBHIP_REQUIRE(n >= 0 && max_step >= 4, "bad sizes");
     if (n == 0) return EOBHIP_OK;
+    rc = ensure_qnm_table(stream);
+    if (rc) return rc;
     const BinInputs in{m1, m2, chi1, chi2};
     const long long nq = (long long)n * (nm + 1);
     hipLaunchKernelGGL(k_imr_qnm, dim3(blocks_for(nq, 64)), dim3(64), 0, as_stream(stream), in, dense, dT_M, n, max_step, mm, qnm, binf,

@@ -394,7 +394,9 @@ int eobhip_imr_fit(int which, int l, int m, const double* x1, const double* x2, 
                    void* stream);
 /* Kerr QNM frequency M omega_lm0 (e^{-i omega t}, Im < 0; omega complex [n]) of holes of spin af[n]:
  * Leaver's continued fractions, replacing lalsimulation's SimIMREOBGenerateQNMFreqV2fromFinal
- * behind compute_QNM (eob/eobstuff.py:32-41, which returns conj(omega) / M_f). 2 <= l <= 5. */
+ * behind compute_QNM (eob/eobstuff.py:32-41, which returns conj(omega) / M_f).  The fundamental
+ * mode of l = max(2, |m|) <= 5; NaN for |af| > 0.9999.  The first call on a device fills a
+ * table of the ringdown modes' roots (199 spins) and waits for it. */
 int eobhip_kerr_qnm(int l, int m, const double* af, int n, double* omega, void* stream);
 /* The attachment's parameters for n binaries and nm <= 8 modes (ls, ms host arrays; each
  * mode one of (2,2), (2,1), (3,3), (4,4), (5,5)), compute_full_waveform
