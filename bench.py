@@ -549,9 +549,9 @@ def workload_imr(args):
         b.hlm_modes_knots(P.REFERENCE_MODES)
         b.build_mode_splines()
         ev[3].record()
-        b.imr_params(M, dt)
+        prm = b.imr_params(M, dt)
         ev[4].record()
-        hp, hc = b.imr_template(M, dt, N)
+        hp, hc = b.imr_template(M, dt, N, params=prm)
         ev[5].record()
         torch.cuda.synchronize()
         if it >= args.warmup:

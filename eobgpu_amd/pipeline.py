@@ -378,6 +378,12 @@ class EOBBatch:
         ls, ms = (list(x) for x in self._wave_key)
         nm = len(ls)
         L = lib()
+        m1, m2, c1, c2 = self.add[:4].cpu().numpy()
+        same = (np.abs(m1 - m2) < 1e-4) & (((np.abs(c1) < 1e-4) & (np.abs(c2) < 1e-4)) | (np.abs(c1 - c2) < 1e-4))
+        if same.any():
+            # equal masses with equal spins: the odd-m modes vanish and the NQC
+            # systems are singular; the reference stops there too (:2391-2395)
+            raise NotImplementedError(f"binaries {np.flatnonzero(same).tolist()} have equal masses and spins")
         self.build_rp_splines()
         M = torch.as_tensor(np.broadcast_to(np.asarray(total_mass_msun, dtype=np.float64), (self.n,)).copy(), device=self.device)
         dT_M = (dt / (M * MTSUN_SI)).contiguous()
@@ -393,14 +399,15 @@ class EOBBatch:
         self.imr = out
         return out
 
-    def imr_template(self, total_mass_msun, dt, N, amp=None, batch_ringdown=False):
+    def imr_template(self, total_mass_msun, dt, N, amp=None, batch_ringdown=False, params=None):
         """(h+, hx) [n][N]: the IMR template -- amp times the sum over the modes of
         the NQC-corrected modes up to the attachment sample and of the ringdown
         after it (compute_full_waveform, eob/seobnrv4phm.py:2308-2556, summed as
         BBHWaveformTD does, :720) on t_j = t_0 + j dt seconds, j < N.  Each binary's
         ringdown lasts its own 30 (2,2) damping times; batch_ringdown=True pads every
-        binary to the batch's longest, as the reference does (:2514)."""
-        p = self.imr_params(total_mass_msun, dt)
+        binary to the batch's longest, as the reference does (:2514).  params: the
+        result of ``imr_params`` for the same masses and dt (computed here if None)."""
+        p = self.imr_params(total_mass_msun, dt) if params is None else params
         if batch_ringdown:
             p["bini"][:, 2] = p["bini"][:, 2].max()
         ls, ms = (list(x) for x in self._wave_key)

@@ -181,3 +181,41 @@ def test_nqc_factor_and_ringdown_dropins(OI, imr_case):
     hring, _ = ES.compute_MR_mode_free(t, np.array([m1s]), np.array([m2s]), np.array([p["chi1"][0]]), np.array([p["chi2"][0]]), ap, ells,
                                        mms, t_match=np.array([d["t_match"]]), phi_match=phi)
     np.testing.assert_allclose(hring[0], d["hring"], rtol=1e-9, atol=1e-12 * np.abs(d["hring"]).max())
+
+
+def test_equal_mass_equal_spin_is_refused(dev):
+    """The reference raises for equal masses with equal (or no) spins
+    (eob/seobnrv4phm.py:2391-2395): the odd-m modes vanish there."""
+    m = np.array([20.0, 30.0]); c = np.array([0.3, 0.1])
+    b = P.EOBBatch(P.make_additional_args(m, m.copy(), c, c.copy()), max_step=1024, device=dev)
+    b.initial_conditions(np.full(2, 10.0))
+    b.integrate(rtol=1e-10, atol=1e-10, two_pass=True)
+    b.hlm_modes_knots(MODES)
+    b.build_mode_splines()
+    with pytest.raises(NotImplementedError):
+        b.imr_params(2 * m, 1.0 / 16384)
+
+
+def test_imr_template_22_only(OI, dev):
+    """The attachment with the (2,2) mode alone (the reference's default SEOBNRv4
+    configuration), against the oracle on the device's own knots."""
+    p = configs.small_batch(n=2, seed=31)
+    b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=4096, device=dev)
+    b.initial_conditions(np.full(2, 10.5))
+    b.integrate(rtol=1e-10, atol=1e-10, two_pass=True)
+    b.hlm_modes_knots(((2, 2),))
+    b.build_mode_splines()
+    M = p["m1"] + p["m2"]
+    dt = 1.0 / 16384
+    N = 1 << 14
+    hp, hc = b.imr_template(M, dt, N)
+    h = (hp + 1j * hc).cpu().numpy()
+    ns = b.num_steps.cpu().numpy()
+    dense = b.dense.cpu().numpy()
+    kn = b.mode_knots.cpu().numpy()
+    dTM = b.imr["dT_M"].cpu().numpy()
+    for i in range(2):
+        L = int(ns[i])
+        tot = OI.imr_template(dense[i, :L, 0], dense[i, :L, 1], dense[i, :L, 2], dense[i, :L, 3], kn[i, :L, :2].T.copy(), ((2, 2),),
+                              p["m1"][i], p["m2"][i], p["chi1"][i], p["chi2"][i], float(dTM[i]), N)
+        assert np.max(np.abs(h[i] - tot)) < 1e-9 * np.abs(tot).max(), i
