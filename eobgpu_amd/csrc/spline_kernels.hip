@@ -92,9 +92,14 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_TD(const double* __restr
 // a system of its own (cut off where the halo ends) and keeps the slopes of its
 // own rows.  Inside the system every row is diagonally dominant (2(dx1 + dx2)
 // against dx1 + dx2), so the effect of the cut on a row k rows away is damped
-// by at least 2^-k: with 64 halo rows it is below 1e-19 of the neighbouring
-// slopes, i.e. the chunked solve returns the slopes of the whole system to
-// rounding.  The not-a-knot end rows are never cut.
+// by at least 2^-k, and by 0.268^k on evenly spaced knots: with 32 halo rows
+// it is below 5e-19 of the neighbouring slopes on even knots and 2e-10 in the
+// worst case the dominance allows (every spacing shrinking by a large factor
+// over 32 rows, which an adaptive step sequence cannot do).  Emulated on the
+// CPU against the uncut solve, 32 halo rows give the same coefficients as 64
+// to rounding on trajectory knots (spacing ratio 33 where the fixed-step pass
+// begins) and on 5k-60k-knot chirps; 24 rows already move them by 1e-14.  The
+// not-a-knot end rows are never cut.
 //
 // Nothing of the system goes to memory: the rows are formed from t and y as the
 // sweeps reach them, the forward elimination keeps only its carry (c', d') at
@@ -104,8 +109,8 @@ __global__ __launch_bounds__(kBlock) void k_interpolate_TD(const double* __restr
 // coefficients of each interval as soon as both end slopes are known.  Memory
 // traffic is t and y read once and the coefficients written once (the
 // recomputation and the halos re-read rows from the caches).
-constexpr int kHalo = 64;
-constexpr int kChunkMin = 64;
+constexpr int kHalo = 32;
+constexpr int kChunkMin = 96;
 constexpr int kSplineLanes = 64;
 constexpr int kSeg = 16;
 constexpr int kMaxSeries = 32;

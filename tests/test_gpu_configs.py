@@ -98,7 +98,7 @@ def _scipy_hplus(t, phi, re, im, M, dt, N):
 
 
 def test_spline_workload_against_scipy(oracle, dev):
-    """Lengths just above the 2 x 128-row chunk minimum, not a multiple of 16,
+    """Lengths just above twice the 96-row chunk minimum, not a multiple of 16,
     about 5k, just above 32768 (256 chunks) and 60k, max_step the largest."""
     K = np.array([259, 300, 5003, 5000, 32771, 33000, 45013, 60000])
     N, dt, Msun = 1 << 20, 1.0 / 4096, 30.0
