@@ -316,8 +316,8 @@ def time_graph(pipe, steps, warmup):
 
 
 def cpu_info():
-    """(threads this process may use, capped at the 16 host cores one GPU's share
-    has on the GPU box; CPU model)."""
+    """(threads this process may use, capped at 16 -- the host cores that go with
+    one GPU on an 8-GPU MI355X node; CPU model)."""
     import os
     n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     model = "unknown"

@@ -6,8 +6,9 @@ Same function names and arguments.  Every fit is evaluated by one elementwise
 HIP kernel (``eobhip_imr_fit``), the QNM frequencies by Leaver's continued
 fractions on the device (``eobhip_kerr_qnm``); the few elementwise
 combinations around them run as torch operations on the GPU.  Inputs may be
-numpy arrays / scalars or torch tensors: numpy in gives numpy out, a CUDA
-tensor in gives a CUDA tensor out.
+numpy arrays / scalars, torch tensors or other GPU arrays (objects with
+``__cuda_array_interface__``, such as the cupy arrays the reference passes):
+numpy in gives numpy out, a GPU array in gives a CUDA torch tensor out.
 
 Two functions replace libraries the reference imports and this image lacks:
 ``bbh_final_spin_non_precessing_HBR2016`` (pesummary) is the published HBR2016
@@ -37,24 +38,46 @@ _FIT = dict(DT22=0, AMP=1, ADOT=2, ADDOT=3, OMEGA=4, OMEGADOT=5, C1F=6, C2F=7, D
 _DEV = "cuda"
 
 
+def _on_device(x):
+    """True for torch tensors and for GPU arrays of other libraries (objects with
+    __cuda_array_interface__, e.g. the cupy arrays the reference passes)."""
+    return isinstance(x, torch.Tensor) or hasattr(x, "__cuda_array_interface__")
+
+
 def _dev(x, shape=None):
-    t = x.to(device=_DEV, dtype=torch.float64) if isinstance(x, torch.Tensor) else torch.as_tensor(
-        np.asarray(x, dtype=np.float64), device=_DEV)
+    if isinstance(x, torch.Tensor):
+        t = x.to(device=_DEV, dtype=torch.float64)
+    elif hasattr(x, "__cuda_array_interface__"):
+        t = torch.as_tensor(x, device=_DEV).to(torch.float64)
+    else:
+        t = torch.as_tensor(np.asarray(x, dtype=np.float64), device=_DEV)
     if shape is not None:
         t = t.expand(shape)
     return t.contiguous()
 
 
+def _shape(x):
+    return tuple(x.shape) if (isinstance(x, torch.Tensor) or hasattr(x, "__cuda_array_interface__")) else np.shape(x)
+
+
+def _host(x):
+    """x as a host numpy array (mode lists, per-mode grouping)."""
+    if isinstance(x, torch.Tensor):
+        return x.detach().cpu().numpy()
+    if hasattr(x, "__cuda_array_interface__"):
+        return torch.as_tensor(x, device=_DEV).cpu().numpy()
+    return np.asarray(x)
+
+
 def _back(t, like):
-    if any(isinstance(x, torch.Tensor) for x in like):
+    if any(_on_device(x) for x in like):
         return t
     out = t.cpu().numpy()
     return out if out.ndim else float(out)
 
 
 def _fit(which, l, m, *xs):
-    shape = torch.broadcast_shapes(*[torch.as_tensor(np.asarray(x)).shape if not isinstance(x, torch.Tensor) else x.shape
-                                     for x in xs])
+    shape = torch.broadcast_shapes(*[_shape(x) for x in xs])
     ts = [_dev(x, shape) for x in xs] + [None] * (4 - len(xs))
     out = torch.empty(shape, dtype=torch.float64, device=_DEV)
     n = out.numel()
@@ -65,8 +88,8 @@ def _fit(which, l, m, *xs):
 
 def _per_mode(which, ell, m, eta, chi):
     """[nbin][nmodes] for mode lists ell, m (the reference's layout, eob/eobstuff.py:108)."""
-    ell = np.atleast_1d(np.asarray(ell.cpu() if isinstance(ell, torch.Tensor) else ell)).astype(int)
-    m = np.atleast_1d(np.asarray(m.cpu() if isinstance(m, torch.Tensor) else m)).astype(int)
+    ell = np.atleast_1d(_host(ell)).astype(int)
+    m = np.atleast_1d(_host(m)).astype(int)
     cols = [_fit(which, li, mi, eta, chi) for li, mi in zip(ell, m)]
     if isinstance(cols[0], torch.Tensor):
         return torch.stack(cols, dim=-1)
@@ -94,12 +117,12 @@ def compute_QNM(ell, m, n, af, Mf):
     if int(n) != 0:
         raise NotImplementedError("only the fundamental mode (n = 0) is provided")
     like = (af, Mf)
-    shape = torch.broadcast_shapes(*[x.shape if isinstance(x, torch.Tensor) else np.shape(x) for x in like])
+    shape = torch.broadcast_shapes(*[_shape(x) for x in like])
     a = _dev(af, shape)
     out = torch.empty(tuple(shape) + (2,), dtype=torch.float64, device=_DEV)
     call("eobhip_kerr_qnm", int(ell), int(m), a.data_ptr(), a.numel(), out.data_ptr(), current_stream())
     w = torch.complex(out[..., 0], out[..., 1]) / _dev(Mf, shape)
-    if any(isinstance(x, torch.Tensor) for x in like):
+    if any(_on_device(x) for x in like):
         return w
     w = w.cpu().numpy()
     return w if w.ndim else complex(w)
@@ -127,10 +150,7 @@ def EOBCalculateRDPhaseCoefficient2(ell, m, eta, chi, xp=None):
 
 
 def _ew(*xs):
-    like = xs
-    ts = [x.to(device=_DEV, dtype=torch.float64) if isinstance(x, torch.Tensor) else torch.as_tensor(np.asarray(x, np.float64), device=_DEV)
-          for x in xs]
-    return ts, like
+    return [_dev(x) for x in xs], xs
 
 
 def EOBCalculateRDAmplitudeConstraintedCoefficient1(c1f, c2f, sigmaR, amp, damp, eta, xp=None):
@@ -154,19 +174,18 @@ def EOBCalculateRDPhaseConstraintedCoefficient1(d1f, d2f, sigmaI, omega, xp=None
 # ------------------------------------------------------------ NR peak fits
 def _vec(fn):
     def per_mode(ell, m, *args):
-        ell_a, m_a = np.asarray(ell.cpu() if isinstance(ell, torch.Tensor) else ell), np.asarray(
-            m.cpu() if isinstance(m, torch.Tensor) else m)
+        ell_a, m_a = _host(ell), _host(m)
         if ell_a.ndim == 0:
             return fn(int(ell_a), int(m_a), *args)
         shape = np.broadcast_shapes(ell_a.shape, *[tuple(a.shape) if hasattr(a, "shape") else () for a in args])
         el, mm = np.broadcast_to(ell_a, shape).ravel(), np.broadcast_to(m_a, shape).ravel()
-        flat = [np.broadcast_to(np.asarray(a.cpu() if isinstance(a, torch.Tensor) else a, np.float64), shape).ravel() for a in args]
+        flat = [np.broadcast_to(_host(a).astype(np.float64), shape).ravel() for a in args]
         out = np.empty(el.size)
         for key in set(zip(el.tolist(), mm.tolist())):
             sel = (el == key[0]) & (mm == key[1])
             out[sel] = np.asarray(fn(key[0], key[1], *[f[sel] for f in flat]))
         out = out.reshape(shape)
-        if any(isinstance(a, torch.Tensor) for a in args):
+        if any(_on_device(a) for a in args):
             return torch.as_tensor(out, device=_DEV)
         return out
 
@@ -216,16 +235,14 @@ def EOBNonQCCorrection(r, phi, pr, pphi, omega, coeffs, xp=None):
     """eob/seobnrv4phm.py:1686-1700: the complex NQC factor from the coefficients
     dict (a1 a2 a3 a3S a4 a5 b1 b2 b3 b4)."""
     (r, pr, om), like = _ew(r, pr, omega)
-    c = {k: (v.to(device=_DEV, dtype=torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, np.float64),
-                                                                                                         device=_DEV))
-         for k, v in coeffs.items()}
+    c = {k: _dev(v) for k, v in coeffs.items()}
     sqrtR = torch.sqrt(r)
     rO = r * om
     mag = 1.0 + (pr * pr / (rO * rO)) * (c["a1"] + c["a2"] / r + (c["a3"] + c["a3S"]) / (r * sqrtR) + c["a4"] / (r * r)
                                          + c["a5"] / (r * r * sqrtR))
     ph = c["b1"] * pr / rO + pr * pr * pr / rO * (c["b2"] + c["b3"] / sqrtR + c["b4"] / r)
     out = torch.complex(mag * torch.cos(ph), mag * torch.sin(ph))
-    return out if any(isinstance(x, torch.Tensor) for x in like) else out.cpu().numpy()
+    return out if any(_on_device(x) for x in like) else out.cpu().numpy()
 
 
 def compute_MR_mode_free(t, m1, m2, chi1, chi2, attach_params, ell, m, t_match=0, phi_match=0, debug=False, xp=None):
@@ -233,15 +250,13 @@ def compute_MR_mode_free(t, m1, m2, chi1, chi2, attach_params, ell, m, t_match=0
     phase, from the attachment values (amp, damp, omega [nbin][nmodes]) and the remnant
     (final_mass, final_spin [nbin])."""
     (t, m1, m2, chi1, chi2), like = _ew(t, m1, m2, chi1, chi2)
-    ap = {k: (v.to(device=_DEV, dtype=torch.float64) if isinstance(v, torch.Tensor) else torch.as_tensor(np.asarray(v, np.float64),
-                                                                                                         device=_DEV))
-          for k, v in attach_params.items()}
+    ap = {k: _dev(v) for k, v in attach_params.items()}
     t_match = _dev(t_match)
     phi_match = _dev(phi_match)
     eta = m1 * m2 / (m1 + m2) ** 2
     chi = 0.5 * (chi1 + chi2) + 0.5 * (chi1 - chi2) * (m1 - m2) / (m1 + m2) / (1.0 - 2.0 * eta)
-    ell_a = np.atleast_1d(np.asarray(ell.cpu() if isinstance(ell, torch.Tensor) else ell)).astype(int)
-    m_a = np.atleast_1d(np.asarray(m.cpu() if isinstance(m, torch.Tensor) else m)).astype(int)
+    ell_a = np.atleast_1d(_host(ell)).astype(int)
+    m_a = np.atleast_1d(_host(m)).astype(int)
     w = torch.stack([compute_QNM(int(li), int(mi), 0, ap["final_spin"], ap["final_mass"]).conj()
                      for li, mi in zip(ell_a, m_a)], dim=-1)   # lalsimulation's convention
     sigmaR, sigmaI = -w.imag, -w.real
@@ -261,6 +276,6 @@ def compute_MR_mode_free(t, m1, m2, chi1, chi2, attach_params, ell, m, t_match=0
         (1 + d2f[:, :, None] * torch.exp(-d1f[:, :, None] * x)) / (1 + d2f[:, :, None]))
     tw = torch.complex(-w.real, w.imag)
     hlm = eta[:, None, None] * Alm * torch.exp(1j * philm) * torch.exp(1j * tw[:, :, None] * x)
-    if any(isinstance(v, torch.Tensor) for v in like):
+    if any(_on_device(v) for v in like):
         return hlm, philm
     return hlm.cpu().numpy(), philm.cpu().numpy()

@@ -219,3 +219,26 @@ def test_imr_template_22_only(OI, dev):
         tot = OI.imr_template(dense[i, :L, 0], dense[i, :L, 1], dense[i, :L, 2], dense[i, :L, 3], kn[i, :L, :2].T.copy(), ((2, 2),),
                               p["m1"][i], p["m2"][i], p["chi1"][i], p["chi2"][i], float(dTM[i]), N)
         assert np.max(np.abs(h[i] - tot)) < 1e-9 * np.abs(tot).max(), i
+
+
+class _GpuArray:
+    """A GPU array of another library: only __cuda_array_interface__ and shape."""
+
+    def __init__(self, t):
+        self._t = t
+        self.__cuda_array_interface__ = t.__cuda_array_interface__
+        self.shape = tuple(t.shape)
+
+
+def test_dropins_take_other_gpu_arrays(dev):
+    rng = np.random.default_rng(7)
+    m1, m2 = rng.uniform(0.3, 0.7, 5), rng.uniform(0.3, 0.7, 5)
+    c1, c2 = rng.uniform(-0.5, 0.5, 5), rng.uniform(-0.5, 0.5, 5)
+    want = ES.compute_final_mass_SEOBNRv4(m1, m2, c1, c2)
+    got = ES.compute_final_mass_SEOBNRv4(*[_GpuArray(torch.tensor(x, device=dev)) for x in (m1, m2, c1, c2)])
+    assert isinstance(got, torch.Tensor)
+    np.testing.assert_array_equal(got.cpu().numpy(), want)
+    eta = m1 * m2 / (m1 + m2) ** 2
+    got = ES.EOBCalculateRDAmplitudeCoefficient1(np.array([2, 3]), np.array([2, 3]), _GpuArray(torch.tensor(eta, device=dev)),
+                                                 _GpuArray(torch.tensor(c1, device=dev)))
+    np.testing.assert_array_equal(got.cpu().numpy(), ES.EOBCalculateRDAmplitudeCoefficient1(np.array([2, 3]), np.array([2, 3]), eta, c1))
