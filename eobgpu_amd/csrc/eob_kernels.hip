@@ -663,6 +663,11 @@ static_assert(sizeof(BinData) % 16 == 0 && kStageBytesPerWave % 16 == 0, "16-byt
 
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) {
+    // The integrating wavefronts issue first on a SIMD they share with the
+    // downstream kernels of binaries that have already finished (the streamed
+    // pass): the integrator's makespan is the pass time, and the others fill
+    // its issue gaps.
+    __builtin_amdgcn_s_setprio(3);
     // 64-lane groups own a wavefront: each keeps its binary's tables in LDS
     constexpr bool kStage = (G == 64);
     extern __shared__ __align__(16) unsigned char stage_lds[];
