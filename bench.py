@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--sustained", type=int, default=4,
                     help="batches of n fresh binaries integrated back to back in one streamed pass (a SIMD freed by a "
                          "short binary starts the next batch's binaries), reported beside the headline; 0 skips it")
+    ap.add_argument("--no-imr", action="store_true",
+                    help="skip the IMR generator figure (five modes with NQC and ringdown) reported beside the headline")
     ap.add_argument("--no-graph", action="store_true",
                     help="skip timing the pass captured as one HIP graph (pipeline.GraphPass, full-length likelihood)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -521,7 +523,7 @@ def workload_capacity(args):
     print(json.dumps(out))
 
 
-def workload_imr(args):
+def run_imr(args, steps, warmup):
     """The waveform generator of the headline batch: for each of the 1024 uneven
     binaries, fresh parameters ingested on the device, initial conditions, the
     two-pass trajectory, the five modes on the knots, their splines, the NQC and
@@ -533,7 +535,7 @@ def workload_imr(args):
     n = args.n
     N = 1 << args.log2N
     dt = 1.0 / 16384
-    ps = [configs.uneven_batch(n=n, seed=2 + k) for k in range(args.warmup + args.steps)]
+    ps = [configs.uneven_batch(n=n, seed=2 + k) for k in range(warmup + steps)]
     b = P.EOBBatch.from_params(ps[0]["m1"], ps[0]["m2"], ps[0]["chi1"], ps[0]["chi2"], max_step=args.max_step)
     names = ("setup_ic", "trajectory", "knots_splines", "imr_params", "imr_template")
     times, stages = [], []
@@ -554,7 +556,7 @@ def workload_imr(args):
         hp, hc = b.imr_template(M, dt, N, params=prm)
         ev[5].record()
         torch.cuda.synchronize()
-        if it >= args.warmup:
+        if it >= warmup:
             times.append(ev[0].elapsed_time(ev[-1]))
             stages.append([ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))])
         del hp, hc
@@ -564,14 +566,21 @@ def workload_imr(args):
     nqc = b.imr["nqc"].cpu().numpy()
     ok = np.isfinite(nqc).all(axis=(1, 2)) & (bini[:, 1] >= 0)
     out = {"metric": "IMR waveforms/s (1024 uneven binaries, five modes with NQC and ringdown)", "value": n / (ms * 1e-3),
-           "unit": "waveforms/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "unit": "waveforms/s", "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": ms,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
            "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2], r0~U[15,40] M, fresh "
                                   "parameters each step; modes (2,2),(2,1),(3,3),(4,4),(5,5); NQC + ringdown; 2^%d samples at "
                                   "1/16384 s" % args.log2N, "global_batch": n, "seq_len": N},
            "stages_ms": dict(zip(names, [float(x) for x in st])),
            "attached_binaries": int(ok.sum()), "ringdown_samples": {"min": int(bini[ok, 2].min()), "max": int(bini[ok, 2].max())}}
-    print(json.dumps(out))
+    del b
+    torch.cuda.empty_cache()
+    return out
+
+
+def workload_imr(args):
+    """``--workload imr``: the IMR generator's line (run_imr)."""
+    print(json.dumps(run_imr(args, args.steps, args.warmup)))
 
 
 def capacity_served(p, args, budget_gb=6.0, N=1 << 12, dt=1.0 / 2048):
@@ -725,6 +734,14 @@ def main():
     if not args.no_serve:
         served = serve_section(args)
 
+    # the IMR generator on the same kind of batch (five modes, NQC and ringdown)
+    imr = None
+    if not args.no_imr:
+        r = run_imr(args, 2, 1)
+        imr = {"waveforms_per_s": r["value"], "ms_per_step": r["ms_per_step"], "stages_ms": r["stages_ms"],
+               "attached_binaries": r["attached_binaries"], "config": r["config"]["workload"]}
+        print(f"[bench] IMR generator: {json.dumps(imr)}", file=sys.stderr, flush=True)
+
     # roofline of the dominant kernel
     dom = max(stages, key=stages.get)
     total_attempts = float(att.sum())
@@ -771,6 +788,7 @@ def main():
         "attempted_steps_total": int(total_attempts), "stop_codes": {str(k): int(v) for k, v in zip(*np.unique(si, return_counts=True))},
         "logl_finite": bool(np.all(np.isfinite(logl))),
         "full_length_likelihood": full_length, "sustained": sustained, "graph_pass": graph, "plain_batch": plain, "served": served,
+        "imr": imr,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
