@@ -666,8 +666,12 @@ __global__ __launch_bounds__(kBlock) void k_integrate_persistent(IntegParams P) 
     // The integrating wavefronts issue first on a SIMD they share with the
     // downstream kernels of binaries that have already finished (the streamed
     // pass): the integrator's makespan is the pass time, and the others fill
-    // its issue gaps.
-    __builtin_amdgcn_s_setprio(3);
+    // its issue gaps.  Only the one-binary-per-wavefront build takes it: the
+    // lane-per-binary build (G = 1) spills about 2 KB per lane, and with the
+    // priority instruction at its entry its register allocation changed and two
+    // of sixteen fixed-step binaries no longer matched its plain-kernel twin,
+    // which shares the step code (the G = 64, 16 builds stayed bit-identical).
+    if constexpr (G == 64) __builtin_amdgcn_s_setprio(3);
     // 64-lane groups own a wavefront: each keeps its binary's tables in LDS
     constexpr bool kStage = (G == 64);
     extern __shared__ __align__(16) unsigned char stage_lds[];
