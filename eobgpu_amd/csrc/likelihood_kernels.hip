@@ -28,6 +28,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kPerThread = 16;                 // resample: samples per thread per block
 constexpr int kSpan = kBlock * kPerThread;     // samples per block
+constexpr int kSearch = 4;                     // resample: samples per thread whose knot searches run together
+static_assert(kPerThread % kSearch == 0, "kSearch divides kPerThread");
 constexpr int kBinsPerThread = 8;              // hh: bins per thread per block
 constexpr int kBinSpan = kBlock * kBinsPerThread;
 
@@ -158,26 +160,33 @@ __global__ __launch_bounds__(kBlock) void k_lik_resample(const double* __restric
             knot_window<kBlock>(win, T, 5, len, t0 + (double)j0 * dT, t0 + (double)jmax * dT);
         }
         const double a = amp[b];
-#pragma unroll 4
-        for (int u = 0; u < kPerThread; ++u) {
-            const long long j = j0 + (long long)u * kBlock + threadIdx.x;
-            if (j >= M) break;
-            double val = 0.0;
-            if (j < nb) {
-                const double tj = t0 + (double)j * dT;
-                const int ind = knot_interval<kBlock>(win, T, 5, len, tj);
-                const double x = tj - T[5 * ind];
-                const double x2 = x * x, x3 = x2 * x;
-                const double* c = C + 12 * ind;
-                const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
-                const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
-                const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
-                double sp, cp;
-                sincos(2.0 * ph, &sp, &cp);
-                val = a * (sr * cp + si * sp);
-                acc += val * g[j];
+        for (int u0 = 0; u0 < kPerThread; u0 += kSearch) {
+            double tj[kSearch];
+            int ind[kSearch];
+#pragma unroll
+            for (int k = 0; k < kSearch; ++k) tj[k] = t0 + (double)(j0 + (long long)(u0 + k) * kBlock + threadIdx.x) * dT;
+            if (j0 < nb) knot_intervals<kBlock, kSearch>(win, T, 5, len, tj, ind);
+#pragma unroll
+            for (int k = 0; k < kSearch; ++k) {
+                const long long j = j0 + (long long)(u0 + k) * kBlock + threadIdx.x;
+                if (j < M) {
+                    double val = 0.0;
+                    if (j < nb) {
+                        double tk, c[12];
+                        interval_row<12>(T, 5, C, ind[k], tk, c);
+                        const double x = tj[k] - tk;
+                        const double x2 = x * x, x3 = x2 * x;
+                        const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
+                        const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
+                        const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
+                        double sp, cp;
+                        sincos(2.0 * ph, &sp, &cp);
+                        val = a * (sr * cp + si * sp);
+                        acc += val * g[j];
+                    }
+                    out[j] = val;
+                }
             }
-            out[j] = val;
         }
     }
     // every thread of the block reaches the reduction's barrier (the window's

@@ -283,6 +283,8 @@ __global__ __launch_bounds__(kSplineLanes) void k_spline_build_seg(const double*
 // (knot_window) and each thread searches there.
 constexpr int kResPerThread = 16;
 constexpr int kResSpan = kBlock * kResPerThread;
+constexpr int kResSearch = 4;   // samples per thread whose knot searches run together
+static_assert(kResPerThread % kResSearch == 0, "kResSearch divides kResPerThread");
 __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restrict__ dense, const double* __restrict__ coef,
                                                            const int* __restrict__ num_steps, const double* __restrict__ amp,
                                                            const double* __restrict__ dT_M, int n, int max_step, int N,
@@ -316,29 +318,36 @@ __global__ __launch_bounds__(kBlock) void k_resample_hplus(const double* __restr
             const long long jmax = (j0 + kResSpan < nb ? j0 + kResSpan : nb) - 1;
             knot_window<kBlock>(win, T, 5, len, t0 + (double)j0 * dT, t0 + (double)jmax * dT);
         }
-#pragma unroll 4
-        for (int u = 0; u < kResPerThread; ++u) {
-            const long long j = j0 + (long long)u * kBlock + threadIdx.x;
-            if (j >= N) break;
-            double val = 0.0, valc = 0.0;
-            if (j < nb) {
-                const double tj = t0 + (double)j * dT;
-                const int ind = knot_interval<kBlock>(win, T, 5, len, tj);
-                const double x = tj - T[5 * ind];
-                const double x2 = x * x, x3 = x2 * x;
-                const double* c = C + 12 * ind;
-                const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
-                const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
-                const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
-                double sp, cp;
-                sincos(2.0 * ph, &sp, &cp);
-                // the template (sr + i si)(cos 2ph - i sin 2ph); h+ and hx are its real and
-                // imaginary parts, the series the reference transforms (eob/seobnrv4phm.py:720, 742-744)
-                val = a * (sr * cp + si * sp);
-                valc = a * (si * cp - sr * sp);
+        for (int u0 = 0; u0 < kResPerThread; u0 += kResSearch) {
+            double tj[kResSearch];
+            int ind[kResSearch];
+#pragma unroll
+            for (int k = 0; k < kResSearch; ++k) tj[k] = t0 + (double)(j0 + (long long)(u0 + k) * kBlock + threadIdx.x) * dT;
+            if (j0 < nb) knot_intervals<kBlock, kResSearch>(win, T, 5, len, tj, ind);
+#pragma unroll
+            for (int k = 0; k < kResSearch; ++k) {
+                const long long j = j0 + (long long)(u0 + k) * kBlock + threadIdx.x;
+                if (j < N) {
+                    double val = 0.0, valc = 0.0;
+                    if (j < nb) {
+                        double tk, c[12];
+                        interval_row<12>(T, 5, C, ind[k], tk, c);
+                        const double x = tj[k] - tk;
+                        const double x2 = x * x, x3 = x2 * x;
+                        const double ph = c[0] + c[1] * x + c[2] * x2 + c[3] * x3;
+                        const double sr = c[4] + c[5] * x + c[6] * x2 + c[7] * x3;
+                        const double si = c[8] + c[9] * x + c[10] * x2 + c[11] * x3;
+                        double sp, cp;
+                        sincos(2.0 * ph, &sp, &cp);
+                        // the template (sr + i si)(cos 2ph - i sin 2ph); h+ and hx are its real and
+                        // imaginary parts, the series the reference transforms (eob/seobnrv4phm.py:720, 742-744)
+                        val = a * (sr * cp + si * sp);
+                        valc = a * (si * cp - sr * sp);
+                    }
+                    out[j] = val;
+                    if (outc) outc[j] = valc;
+                }
             }
-            out[j] = val;
-            if (outc) outc[j] = valc;
         }
         __syncthreads();   // the window is rewritten by the next span
     }
