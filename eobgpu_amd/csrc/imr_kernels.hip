@@ -95,8 +95,10 @@ __device__ __forceinline__ int knot_index(const double* __restrict__ T, int len,
 struct Piece {
     double x, x2, x3;
     bool last;
-    __device__ double v(const double* c) const { return last ? c[0] : c[0] + c[1] * x + c[2] * x2 + c[3] * x3; }
-    __device__ double d(const double* c) const { return last ? 0.0 : c[1] + 2.0 * c[2] * x + 3.0 * c[3] * x2; }
+    template <class Ptr>
+    __device__ double v(Ptr c) const { return last ? c[0] : c[0] + c[1] * x + c[2] * x2 + c[3] * x3; }
+    template <class Ptr>
+    __device__ double d(Ptr c) const { return last ? 0.0 : c[1] + 2.0 * c[2] * x + 3.0 * c[3] * x2; }
 };
 
 // numpy.mod for a positive divisor
@@ -522,14 +524,59 @@ __device__ __forceinline__ void sincos_nqc(double x, double* s, double* c) {
 // for idx < j < idx + nadd, 0 after), t_j = t_0 + j dT; hplus = Re, hcross = Im.
 constexpr int kImrBlock = 256;
 constexpr int kImrSpan = 2048;
+
+// One inspiral sample's sum over the NM modes: row holds the interval's
+// spline coefficients (Re of each mode, Im of each mode, phi), rpr those of r
+// and pr; nq the binary's NQC coefficients (5 per mode).  e^{-i m phi} is
+// formed from e^{-i phi} by m - 1 products (the same products as a table of
+// its powers, without selecting from the table per mode).
+template <int NM, class Ptr>
+__device__ __forceinline__ void imr_inspiral_sample(Ptr row, Ptr rpr, const Piece& p, const ImrModes& mm, const double* __restrict__ nq,
+                                                    double& hr, double& hi) {
+    const double phi = p.v(row + 4 * (2 * NM));
+    const double om = p.d(row + 4 * (2 * NM));
+    const double r = p.v(rpr), pr = p.v(rpr + 4);
+    const double sr = sqrt(r), rO = r * om;
+    const double P = pr * pr / (rO * rO);
+    const double ir = 1.0 / r, irs = 1.0 / (r * sr);
+    const double p1 = pr / rO, p3 = pr * pr * pr / rO;
+    double sp, cph;
+    sincos(phi, &sp, &cph);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) {
+        const double re = p.v(row + 4 * q), im = p.v(row + 4 * (NM + q));
+        double pr_ = cph, pi_ = -sp;   // e^{-i phi}
+        for (int k = 1; k < mm.m[q]; ++k) {
+            const double t = pr_ * cph + pi_ * sp;
+            pi_ = pi_ * cph - pr_ * sp;
+            pr_ = t;
+        }
+        const double h_r = re * pr_ - im * pi_, h_i = re * pi_ + im * pr_;
+        const double* c = nq + 5 * q;
+        const double mag = 1.0 + P * (c[0] + c[1] * ir + c[2] * irs);
+        double sn, cn;
+        sincos_nqc(c[3] * p1 + p3 * c[4], &sn, &cn);
+        const double fr = mag * cn, fi = mag * sn;
+        hr += h_r * fr - h_i * fi;
+        hi += h_r * fi + h_i * fr;
+    }
+}
+
+// grid (blocks, binaries); a block takes spans of kImrSpan samples, stages the
+// knot times of their inspiral part in LDS (knot_window) and searches there.
+// When all lanes of a wavefront sample one knot interval, its coefficient
+// rows are read once through the scalar cache (constant address space: the
+// tables are read-only while the kernel runs).
+template <int NM>
 __global__ __launch_bounds__(kImrBlock) void k_imr_resample(const double* __restrict__ dense, const int* __restrict__ num_steps,
                                                             const double* __restrict__ coef, const double* __restrict__ rp,
                                                             const double* __restrict__ amp, const double* __restrict__ dT_M, int max_step,
                                                             int N, ImrModes mm, const double* __restrict__ nqc, const double* __restrict__ rd,
                                                             const double* __restrict__ binf, const int* __restrict__ bini,
                                                             double* __restrict__ hplus, double* __restrict__ hcross, int* __restrict__ nsamp) {
+    typedef const __attribute__((address_space(4))) double* ConstPtr;
+    constexpr int cs = 4 * (2 * NM + 1);
     const int b = blockIdx.y;
-    const int nm = mm.nm, cs = 4 * (2 * nm + 1);
     const int len = num_steps[b];
     const int idx = bini[(size_t)b * kBinICount + kBinIdx];
     const int nadd = bini[(size_t)b * kBinICount + kBinNadd];
@@ -540,6 +587,7 @@ __global__ __launch_bounds__(kImrBlock) void k_imr_resample(const double* __rest
     const double* T = dense + (size_t)b * max_step * 5;
     const double* C = coef + (size_t)b * max_step * cs;
     const double* RP = rp + (size_t)b * max_step * 8;
+    const double* nq = nqc + (size_t)b * NM * 5;
     const double t0 = T[0];
     const double dT = dT_M[b];
     const double a = amp[b];
@@ -561,50 +609,25 @@ __global__ __launch_bounds__(kImrBlock) void k_imr_resample(const double* __rest
             if (nvalid > 0 && j <= idx) {
                 const double tj = grid_t(j, dT, t0);
                 const int ind = knot_interval<kImrBlock>(win, T, 5, len, tj);
+                const int i0 = __builtin_amdgcn_readfirstlane(ind);
                 Piece p;
-                p.x = tj - T[5 * ind];
-                p.x2 = p.x * p.x;
-                p.x3 = p.x2 * p.x;
                 p.last = (ind == len - 1);
-                const double* row = C + (size_t)cs * ind;
-                const double phi = p.v(row + 4 * (2 * nm));
-                const double om = p.d(row + 4 * (2 * nm));
-                const double r = p.v(RP + (size_t)8 * ind), pr = p.v(RP + (size_t)8 * ind + 4);
-                const double sr = sqrt(r), rO = r * om;
-                const double P = pr * pr / (rO * rO);
-                const double ir = 1.0 / r, irs = 1.0 / (r * sr);
-                const double p1 = pr / rO, p3 = pr * pr * pr / rO;
-                // e^{-i k phi}, k = 1 .. 8, as powers of e^{-i phi}
-                double sp, cph;
-                sincos(phi, &sp, &cph);
-                double er[kImrMaxM + 1], ei[kImrMaxM + 1];
-                er[0] = 1.0; ei[0] = 0.0;
-#pragma unroll
-                for (int k = 1; k <= kImrMaxM; ++k) {
-                    er[k] = er[k - 1] * cph + ei[k - 1] * sp;
-                    ei[k] = ei[k - 1] * cph - er[k - 1] * sp;
-                }
-                for (int q = 0; q < nm; ++q) {
-                    const double re = p.v(row + 4 * q), im = p.v(row + 4 * (nm + q));
-                    const int km = mm.m[q];
-                    double pr_ = 1.0, pi_ = 0.0;
-#pragma unroll
-                    for (int kk = 1; kk <= kImrMaxM; ++kk)
-                        if (kk == km) { pr_ = er[kk]; pi_ = ei[kk]; }
-                    const double h_r = re * pr_ - im * pi_, h_i = re * pi_ + im * pr_;
-                    const double* nq = nqc + ((size_t)b * nm + q) * 5;
-                    const double mag = 1.0 + P * (nq[0] + nq[1] * ir + nq[2] * irs);
-                    double sn, cn;
-                    sincos_nqc(nq[3] * p1 + p3 * nq[4], &sn, &cn);
-                    const double fr = mag * cn, fi = mag * sn;
-                    hr += h_r * fr - h_i * fi;
-                    hi += h_r * fi + h_i * fr;
+                if (__all(ind == i0)) {
+                    p.x = tj - ((ConstPtr)T)[5 * (size_t)i0];
+                    p.x2 = p.x * p.x;
+                    p.x3 = p.x2 * p.x;
+                    imr_inspiral_sample<NM>((ConstPtr)(C + (size_t)cs * i0), (ConstPtr)(RP + (size_t)8 * i0), p, mm, nq, hr, hi);
+                } else {
+                    p.x = tj - T[5 * (size_t)ind];
+                    p.x2 = p.x * p.x;
+                    p.x3 = p.x2 * p.x;
+                    imr_inspiral_sample<NM>(C + (size_t)cs * ind, RP + (size_t)8 * ind, p, mm, nq, hr, hi);
                 }
             } else if (nvalid > 0 && j < nvalid) {
                 const long long k = j - idx;
                 const double x = (t_match + (double)k * dt) - t_match;
-                for (int q = 0; q < nm; ++q) {
-                    const double* o = rd + ((size_t)b * nm + q) * kRdCount;
+                for (int q = 0; q < NM; ++q) {
+                    const double* o = rd + ((size_t)b * NM + q) * kRdCount;
                     const double Alm = o[kRdC1c] * tanh(o[kRdC1f] * x + o[kRdC2f]) + o[kRdC2c];
                     const double philm = o[kRdPhi] - o[kRdD1c] * log((1.0 + o[kRdD2f] * exp(-o[kRdD1f] * x)) / (1.0 + o[kRdD2f]));
                     const double mod = o[kRdEta] * Alm * exp(-o[kRdWi] * x);
@@ -721,8 +744,26 @@ int eobhip_imr_resample(const double* dense, const int* num_steps, const double*
     if (n == 0) return EOBHIP_OK;
     int bx = (int)((N + (long long)kImrSpan - 1) / kImrSpan);
     if (bx > 4096) bx = 4096;
-    hipLaunchKernelGGL(k_imr_resample, dim3(bx, n), dim3(kImrBlock), 0, as_stream(stream), dense, num_steps, coef, rp_coef, amp, dT_M,
-                       max_step, N, mm, nqc, rd, binf, bini, hplus, hcross, nsamp);
+    const dim3 grid(bx, n), block(kImrBlock);
+    hipStream_t st = as_stream(stream);
+#define EOBHIP_IMR_RESAMPLE(NM)                                                                                                             \
+    case NM:                                                                                                                              \
+        hipLaunchKernelGGL(k_imr_resample<NM>, grid, block, 0, st, dense, num_steps, coef, rp_coef, amp, dT_M, max_step, N, mm, nqc, rd,  \
+                           binf, bini, hplus, hcross, nsamp);                                                                             \
+        break;
+    switch (nm) {
+        EOBHIP_IMR_RESAMPLE(1)
+        EOBHIP_IMR_RESAMPLE(2)
+        EOBHIP_IMR_RESAMPLE(3)
+        EOBHIP_IMR_RESAMPLE(4)
+        EOBHIP_IMR_RESAMPLE(5)
+        EOBHIP_IMR_RESAMPLE(6)
+        EOBHIP_IMR_RESAMPLE(7)
+        EOBHIP_IMR_RESAMPLE(8)
+        default: break;   // make_modes has checked 1 <= nm <= kImrMaxModes
+    }
+    static_assert(kImrMaxModes == 8, "one instantiation per mode count");
+#undef EOBHIP_IMR_RESAMPLE
     return check_launch("imr_resample");
 }
 
