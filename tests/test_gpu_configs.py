@@ -99,7 +99,11 @@ def _scipy_hplus(t, phi, re, im, M, dt, N):
 
 def test_spline_workload_against_scipy(oracle, dev):
     """Lengths just above twice the 96-row chunk minimum, not a multiple of 16,
-    about 5k, just above 32768 (256 chunks) and 60k, max_step the largest."""
+    about 5k, just above 32768 (256 chunks) and 60k, max_step the largest.
+    The three short chirps with 33k-60k knots put more than 1024 knots under
+    58 of their 4096-sample spans, so the resample kernels' bisection over the
+    whole knot array (the path taken when a span's knots overflow the LDS
+    window) runs as well as the LDS one."""
     K = np.array([259, 300, 5003, 5000, 32771, 33000, 45013, 60000])
     N, dt, Msun = 1 << 20, 1.0 / 4096, 30.0
     ts, phis, res, ims, K = configs.chirp_modes(n=K.size, seed=31, N=N, dt=dt, Msun=Msun, K=K)
