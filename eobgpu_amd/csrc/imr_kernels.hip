@@ -504,11 +504,18 @@ __global__ __launch_bounds__(64) void k_imr_params(BinInputs in, const double* _
     }
 }
 
-// sin and cos of x: Taylor series through x^17 / x^18 for |x| < 1/2 (remainder
-// below 2e-23), the library's sincos otherwise.  The NQC phase of the
-// inspiral samples is small except near the attachment.
+// sin and cos of x: Taylor series through x^9 / x^10 when every active lane of
+// the wavefront has |x| < 1/16 (first omitted terms below 2.3e-20 relative),
+// through x^17 / x^18 for |x| < 1/2 (below 2e-23), the library's sincos
+// otherwise.  The NQC phase of the inspiral samples is small except near the
+// attachment.
 __device__ __forceinline__ void sincos_nqc(double x, double* s, double* c) {
-    if (fabs(x) < 0.5) {
+    const double ax = fabs(x);
+    if (__all(ax < 0.0625)) {
+        const double x2 = x * x;
+        *s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880)))));
+        *c = 1.0 + x2 * (-0.5 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800)))));
+    } else if (ax < 0.5) {
         const double x2 = x * x;
         *s = x * (1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 + x2 * (-1.0 / 39916800
                  + x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0 + x2 * (1.0 / 355687428096000.0)))))))));
