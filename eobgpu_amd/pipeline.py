@@ -957,6 +957,28 @@ class StreamedPass:
         self._staged.record(torch.cuda.current_stream(self.device))
         self.order = order
 
+    def set_params_device(self, par):
+        """New binaries whose parameters are already in device memory: par is a
+        float64 tensor [7, n] on this pass's device with rows m1, m2, chi1,
+        chi2, r0, total mass (solar masses) and amplitude.  The binaries keep
+        the caller's order (no host work and no host synchronisation: one
+        device copy on the current stream); ``run`` builds the additional
+        args, tables and initial state from them on the device, as after
+        ``set_params``.  With at most one binary per SIMD (n <= 4 x CUs) every
+        binary starts at once, so the order does not change the pass time."""
+        n = self.n
+        if not isinstance(par, torch.Tensor) or par.dtype != torch.float64 or tuple(par.shape) != (7, n):
+            raise ValueError(f"par must be a float64 tensor of shape (7, {n})")
+        if par.device != self._par.device:
+            raise ValueError(f"par must be on {self._par.device}")
+        if getattr(self, "_iota", None) is None:
+            self._iota = torch.arange(n, dtype=torch.int64, device=self._par.device)
+            self._iota_host = np.arange(n, dtype=np.int64)
+        self._par[:7].copy_(par)
+        self._par[7].copy_(self._iota)
+        self.pos.copy_(self._iota)
+        self.order = self._iota_host
+
     def _launch(self, new, slot):
         rows, st, k = self.rows[slot], self.down_streams[slot], new.size
         rows.idx_host[:k].numpy()[:] = new

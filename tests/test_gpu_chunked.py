@@ -155,8 +155,8 @@ def test_params_setup_matches_host(dev):
 
 
 def test_streamed_pass_new_parameters(dev):
-    """StreamedPass.set_params: a pass over a new batch in the same buffers gives
-    that batch's one-batch numbers."""
+    """StreamedPass.set_params and set_params_device: a pass over a new batch in
+    the same buffers gives that batch's one-batch numbers."""
     n = 16
     p = configs.uneven_batch(n=n, seed=14)
     q = configs.uneven_batch(n=n, seed=15)
@@ -180,3 +180,13 @@ def test_streamed_pass_new_parameters(dev):
     np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
     ns = sp.batch.num_steps.cpu().numpy()
     np.testing.assert_array_equal(ns, b.num_steps.cpu().numpy()[sp.order])
+    # the same batch from device memory (set_params_device): the caller's order
+    par = torch.tensor(np.stack([q["m1"], q["m2"], q["chi1"], q["chi2"], q["r0"], Mq, np.full(n, 0.7)]), device=dev)
+    sp.set_params_device(par)
+    got = sp.run().cpu().numpy()
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+    np.testing.assert_array_equal(sp.order, np.arange(n))
+    np.testing.assert_array_equal(sp.batch.num_steps.cpu().numpy(), b.num_steps.cpu().numpy())
+    with pytest.raises(ValueError):
+        sp.set_params_device(par[:6])
