@@ -196,14 +196,16 @@ def test_equal_mass_equal_spin_is_refused(dev):
         b.imr_params(2 * m, 1.0 / 16384)
 
 
-def test_imr_template_22_only(OI, dev):
-    """The attachment with the (2,2) mode alone (the reference's default SEOBNRv4
-    configuration), against the oracle on the device's own knots."""
+@pytest.mark.parametrize("modes", [((2, 2),), ((2, 2), (3, 3)), ((2, 2), (2, 1), (4, 4)), ((2, 2), (3, 3), (2, 1), (5, 5))])
+def test_imr_template_mode_subsets(OI, dev, modes):
+    """The attachment with one to four of the modes (the (2,2) mode alone is the
+    reference's default SEOBNRv4 configuration), against the oracle on the
+    device's own knots: the template kernel is built once per mode count."""
     p = configs.small_batch(n=2, seed=31)
     b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=4096, device=dev)
     b.initial_conditions(np.full(2, 10.5))
     b.integrate(rtol=1e-10, atol=1e-10, two_pass=True)
-    b.hlm_modes_knots(((2, 2),))
+    b.hlm_modes_knots(modes)
     b.build_mode_splines()
     M = p["m1"] + p["m2"]
     dt = 1.0 / 16384
@@ -216,7 +218,8 @@ def test_imr_template_22_only(OI, dev):
     dTM = b.imr["dT_M"].cpu().numpy()
     for i in range(2):
         L = int(ns[i])
-        tot = OI.imr_template(dense[i, :L, 0], dense[i, :L, 1], dense[i, :L, 2], dense[i, :L, 3], kn[i, :L, :2].T.copy(), ((2, 2),),
+        series = kn[i, :L, :2 * len(modes)].T.copy()
+        tot = OI.imr_template(dense[i, :L, 0], dense[i, :L, 1], dense[i, :L, 2], dense[i, :L, 3], series, modes,
                               p["m1"][i], p["m2"][i], p["chi1"][i], p["chi2"][i], float(dTM[i]), N)
         assert np.max(np.abs(h[i] - tot)) < 1e-9 * np.abs(tot).max(), i
 
