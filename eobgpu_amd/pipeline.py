@@ -859,7 +859,8 @@ class StreamedPass:
     """One pass over n binaries, like ``ChunkedPass``, with the downstream work
     started per group of finished binaries instead of per chunk.
 
-    The whole batch, ordered longest expected first, integrates in one
+    The whole batch (ordered longest expected first by ``set_params``; in the
+    caller's order after ``set_params_device``) integrates in one
     persistent launch on its own stream (``eobhip_integrate_signal``), which
     raises a pinned host flag per binary once the binary's rows are in memory.
     The host polls the flags; whenever binaries have finished it gathers their
