@@ -76,10 +76,11 @@ def parse():
                     help="skip timing the pass captured as one HIP graph (pipeline.GraphPass, full-length likelihood)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
-    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr"), default="uneven",
+    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr", "detector"), default="uneven",
                     help="uneven: the headline 1024-binary pipeline; spline: 512 synthetic (2,2) modes with 5k-60k knots "
                          "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator; "
-                         "imr: the headline binaries' IMR templates (five modes, NQC and ringdown) at 16384 Hz")
+                         "imr: the headline binaries' IMR templates (five modes, NQC and ringdown) at 16384 Hz; detector: the "
+                         "headline binaries' detector-frame signals in three interferometers")
     return ap.parse_args()
 
 
@@ -595,6 +596,72 @@ def workload_imr(args):
     print(json.dumps(run_imr(args, args.steps, args.warmup)))
 
 
+def workload_detector(args, nifo=3):
+    """``--workload detector``: the detector-frame signals of the headline batch
+    (BBHWaveformTD's "detector_fd" branch, eob/seobnrv4phm.py:597-634, 742-767):
+    h+ and hx of the 1024 uneven binaries on 2^20 samples at 1/4096 s
+    (resample_hpc), their real FFTs, the antenna patterns and delays of three
+    interferometers for fresh sky positions, and signal [n][3][N/2+1] =
+    (F+ h+~ + Fx hx~) e^{-2 pi i f dt} (k_detector_signal).  The trajectory,
+    knots and splines are built once before the timed steps; the detectors are
+    synthetic (orthonormal arm pairs, vertices on a 6 400 km sphere)."""
+    import torch
+    from eobgpu_amd import configs, detector, pipeline as P
+    n, N, dt = args.n, 1 << args.log2N, 1.0 / 4096
+    F = N // 2 + 1
+    p = configs.uneven_batch(n=n, seed=2)
+    b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=args.max_step)
+    b.initial_conditions(p["r0"])
+    b.integrate(rtol=args.rtol, atol=args.rtol, two_pass=True)
+    b.hlm22_knots()
+    b.build_splines()
+    M = p["m1"] + p["m2"]
+    rng = np.random.default_rng(23)
+    D, V = [], []
+    for _ in range(nifo):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        D.append(0.5 * (np.outer(q[:, 0], q[:, 0]) - np.outer(q[:, 1], q[:, 1])))
+        V.append(q[:, 2] * 6.4e6)
+    D, V = np.array(D), np.array(V)
+    sig = torch.empty((n, nifo, F), dtype=torch.complex128, device=b.device)
+    names = ["hplus_hcross", "rfft", "antenna", "signal"]
+    times, stages = [], []
+    for it in range(args.warmup + args.steps):
+        ra, dec = rng.uniform(0, 2 * np.pi, n), np.arcsin(rng.uniform(-1, 1, n))
+        psi, gmst = rng.uniform(0, np.pi, n), rng.uniform(0, 2 * np.pi, n)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        hp, hc = detector.resample_hpc(b, M, dt, N)
+        ev[1].record()
+        hpf, hcf = P.rfft_batched(hp), P.rfft_batched(hc)
+        ev[2].record()
+        Fp, Fc, tdel = detector.get_detector_information(ra, dec, gmst, psi, D, V, device=b.device)
+        ev[3].record()
+        detector.detector_signal(hpf, hcf, Fp, Fc, tdel, 1.0 / (N * dt), out=sig)
+        ev[4].record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(ev[0].elapsed_time(ev[-1]))
+            stages.append([ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))])
+        del hp, hc, hpf, hcf
+    ms = float(np.mean(times))
+    st = dict(zip(names, [float(x) for x in np.mean(np.array(stages), axis=0)]))
+    sig_bytes = n * F * (2 + nifo) * 16          # h+~ and hx~ read, nifo signals written
+    ach = sig_bytes / (st["signal"] * 1e-3) / 1e9
+    out = {"metric": "detector-frame signals/s (1024 uneven binaries x 3 interferometers)", "value": n / (ms * 1e-3),
+           "unit": "binaries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "1024 aligned-spin binaries (trajectory, knots and splines built once), h+ and hx on 2^%d samples "
+                                  "at 1/4096 s, real FFTs, antenna patterns and delays of 3 synthetic interferometers for fresh "
+                                  "sky positions, detector-frame signals" % args.log2N, "global_batch": n, "seq_len": N},
+           "stages_ms": st,
+           "roofline": {"bound": "hbm", "kernel": "k_detector_signal", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": ach / PEAK_HBM_GBS, "algorithmic": "%d B per (binary, bin): h+~ and hx~ read, 3 signals written" %
+                        ((2 + nifo) * 16)}}
+    print(json.dumps(out))
+
+
 def capacity_served(p, args, budget_gb=6.0, N=1 << 12, dt=1.0 / 2048):
     """The same 16384 binaries as 64 requests of 256 arriving at once, served
     under one memory budget in two ways: admission by projected footprint (size
@@ -659,6 +726,8 @@ def main():
         return workload_capacity(args)
     if args.workload == "imr":
         return workload_imr(args)
+    if args.workload == "detector":
+        return workload_detector(args)
     pipe = Pipeline(args.n, args.log2N, args.max_step, args.rtol)
     pipe.make_data()
     print(f"[bench] data ready, n={args.n} N=2^{args.log2N}", file=sys.stderr, flush=True)

@@ -58,24 +58,55 @@ __global__ __launch_bounds__(kBlock) void k_detector_response(const double* ra, 
 
 // out[b][d][k] = (F+_db h+~_bk + Fx_db hx~_bk) exp(-2 pi i f_k dt_db), f_k = k df,
 // zero below k_min (the reference zeroes the templates below bilby's start bin).
+// grid (blocks, binaries, ceil(nifo / kDetGroup)); a thread reads h+~ and hx~ of
+// its bins once for up to kDetGroup interferometers, and takes bins k0, k0 + S,
+// k0 + 2S, ... (S = the grid's threads per binary, so a wavefront's accesses are
+// consecutive).  The phase e^{-2 pi i f_k dt} comes from one sincos for the
+// thread's first bin and one for the step S df, then a product per bin
+// (about F / S products, each within an ulp or two).
+constexpr int kDetGroup = 4;
+constexpr int kDetBinsPerThread = 32;
 __global__ __launch_bounds__(kBlock) void k_detector_signal(const double2* __restrict__ hp, const double2* __restrict__ hc,
                                                             const double* __restrict__ Fp, const double* __restrict__ Fc,
                                                             const double* __restrict__ tshift, int n, int nifo, int F, double df,
                                                             int k_min, double2* __restrict__ out) {
-    const int b = blockIdx.y, d = blockIdx.z;
-    const double fp = Fp[(size_t)d * n + b], fc = Fc[(size_t)d * n + b], ts = tshift[(size_t)d * n + b];
+    const int b = blockIdx.y, d0 = blockIdx.z * kDetGroup;
+    const int nd = nifo - d0 < kDetGroup ? nifo - d0 : kDetGroup;
+    const long long S = (long long)gridDim.x * blockDim.x;
+    const long long k0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double fp[kDetGroup], fc[kDetGroup], er[kDetGroup], ei[kDetGroup], rr[kDetGroup], ri[kDetGroup];
+#pragma unroll
+    for (int d = 0; d < kDetGroup; ++d) {
+        fp[d] = fc[d] = 0.0;
+        er[d] = rr[d] = 1.0;
+        ei[d] = ri[d] = 0.0;
+        if (d < nd) {
+            const size_t q = (size_t)(d0 + d) * n + b;
+            fp[d] = Fp[q];
+            fc[d] = Fc[q];
+            const double ts = tshift[q];
+            sincos(-2.0 * kPi * ((double)k0 * df) * ts, &ei[d], &er[d]);
+            sincos(-2.0 * kPi * ((double)S * df) * ts, &ri[d], &rr[d]);
+        }
+    }
     const double2* p = hp + (size_t)b * F;
     const double2* c = hc + (size_t)b * F;
-    double2* o = out + ((size_t)b * nifo + d) * F;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < F; k += gridDim.x * blockDim.x) {
-        double2 r = make_double2(0.0, 0.0);
-        if (k >= k_min) {
-            const double sr = fp * p[k].x + fc * c[k].x, si = fp * p[k].y + fc * c[k].y;
-            double s, co;
-            sincos(-2.0 * kPi * ((double)k * df) * ts, &s, &co);
-            r = make_double2(sr * co - si * s, sr * s + si * co);
+    for (long long k = k0; k < F; k += S) {
+        const double2 pk = p[k], ck = c[k];
+#pragma unroll
+        for (int d = 0; d < kDetGroup; ++d) {
+            if (d < nd) {
+                double2 r = make_double2(0.0, 0.0);
+                if (k >= k_min) {
+                    const double sr = fp[d] * pk.x + fc[d] * ck.x, si = fp[d] * pk.y + fc[d] * ck.y;
+                    r = make_double2(sr * er[d] - si * ei[d], sr * ei[d] + si * er[d]);
+                }
+                out[((size_t)b * nifo + d0 + d) * F + k] = r;
+                const double t = er[d] * rr[d] - ei[d] * ri[d];
+                ei[d] = er[d] * ri[d] + ei[d] * rr[d];
+                er[d] = t;
+            }
         }
-        o[k] = r;
     }
 }
 
@@ -97,9 +128,10 @@ int eobhip_detector_signal(const double* hp_fd, const double* hc_fd, const doubl
     EOBHIP_REQUIRE(n >= 0 && nifo >= 0 && F >= 0 && k_min >= 0, "bad sizes");
     EOBHIP_REQUIRE(n <= 65535 && nifo <= 65535, "n and nifo must be <= 65535");
     if (n == 0 || nifo == 0 || F == 0) return EOBHIP_OK;
-    int bx = (int)((F + kBlock - 1) / kBlock);
-    if (bx > 1024) bx = 1024;
-    hipLaunchKernelGGL(k_detector_signal, dim3(bx, n, nifo), dim3(kBlock), 0, as_stream(stream), reinterpret_cast<const double2*>(hp_fd),
+    const long long span = (long long)kBlock * kDetBinsPerThread;
+    const int bx = (int)((F + span - 1) / span);
+    const int bz = (nifo + kDetGroup - 1) / kDetGroup;
+    hipLaunchKernelGGL(k_detector_signal, dim3(bx, n, bz), dim3(kBlock), 0, as_stream(stream), reinterpret_cast<const double2*>(hp_fd),
                        reinterpret_cast<const double2*>(hc_fd), Fplus, Fcross, time_shift, n, nifo, F, df, k_min,
                        reinterpret_cast<double2*>(out));
     return check_launch("detector_signal");
