@@ -76,11 +76,12 @@ def parse():
                     help="skip timing the pass captured as one HIP graph (pipeline.GraphPass, full-length likelihood)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
-    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr", "detector"), default="uneven",
+    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr", "detector", "ic"), default="uneven",
                     help="uneven: the headline 1024-binary pipeline; spline: 512 synthetic (2,2) modes with 5k-60k knots "
                          "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator; "
                          "imr: the headline binaries' IMR templates (five modes, NQC and ringdown) at 16384 Hz; detector: the "
-                         "headline binaries' detector-frame signals in three interferometers")
+                         "headline binaries' detector-frame signals in three interferometers; ic: the reference's "
+                         "initial-condition solvers as drop-ins on the headline binaries")
     return ap.parse_args()
 
 
@@ -662,6 +663,70 @@ def workload_detector(args, nifo=3):
     print(json.dumps(out))
 
 
+def workload_ic(args):
+    """``--workload ic``: the reference's initial-condition solvers as drop-ins on
+    the headline batch (SEOBNRv4PHM's start at f = 20 Hz, eob/seobnrv4phm.py:1966-2070):
+    root_find_all (discrete Newton on IC_cons for r and pphi, src/EOB.cu:2971-3031)
+    then root_find_scalar_all (false position on IC_diss for pr in [-3e-2, 0],
+    :2778-2834), each a synchronous call as the reference's are; beside them the
+    batched circular start at r0 that the pipeline uses (k_initial_conditions)."""
+    import torch
+    from eobgpu_amd import configs, pyEOB, pipeline as P
+    n, dev = args.n, "cuda"
+    p = configs.uneven_batch(n=n, seed=2)
+    omega0 = 20.0 * ((p["m1"] + p["m2"]) * P.MTSUN_SI * np.pi)
+    add = P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"], omega0=omega0)
+
+    def T(x):
+        return torch.tensor(np.ascontiguousarray(x, dtype=np.float64).reshape(-1), device=dev)
+    m1, m2, c1, c2 = add[0], add[1], add[2], add[3]
+    eta = m1 * m2 / (m1 + m2) ** 2
+    chiS, chiA = 0.5 * (c1 + c2), 0.5 * (c1 - c2)
+    ode = pyEOB.SEOBNRv5Class()
+    ode.update_information(T(m1), T(m2), T(eta), T((1 - 2 * eta) * chiS + (m1 - m2) / (m1 + m2) * chiA), T(chiS), T(chiA), 1, n,
+                           T(add[10:80].T))
+    rg = omega0 ** (-2.0 / 3)
+    x0, add0 = T(np.array([rg, np.sqrt(rg)])), T(add)
+    tadd = add0.clone()
+    xo, argbuf = torch.zeros(2 * n, dtype=torch.float64, device=dev), torch.zeros(4 * n, dtype=torch.float64, device=dev)
+    bounds = T(np.tile(np.array([-3e-2, 0.0]), (n, 1)).T)
+    pr = torch.zeros(n, dtype=torch.float64, device=dev)
+    zero = torch.zeros(n, dtype=torch.float64, device=dev)
+    b = P.EOBBatch(add, max_step=64)
+    t_rf, t_sc, t_circ = [], [], []
+    for it in range(args.warmup + args.steps):
+        tadd.copy_(add0)   # IC_diss writes dH/dpphi into the omega row
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pyEOB.root_find_all(xo, x0, argbuf, tadd, 1000, 1e-12, n, 2, 4, 80)
+        t1 = time.perf_counter()
+        x = xo.reshape(2, n)
+        a4 = torch.stack([x[0], zero, zero, x[1]]).reshape(-1).contiguous()
+        ode.root_find_scalar_all(pr, bounds, a4, tadd, 1000, 1e-12, n, 4, 80)
+        t2 = time.perf_counter()
+        b.initial_conditions(p["r0"])
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        if it >= args.warmup:
+            t_rf.append((t1 - t0) * 1e3)
+            t_sc.append((t2 - t1) * 1e3)
+            t_circ.append((t3 - t2) * 1e3)
+    x = xo.reshape(2, n).cpu().numpy()
+    prv = pr.cpu().numpy()
+    ode.deallocate_information()
+    ms = float(np.mean(t_rf) + np.mean(t_sc))
+    out = {"metric": "initial conditions/s (1024 uneven binaries, the reference's solvers as drop-ins)", "value": n / (ms * 1e-3),
+           "unit": "binaries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "1024 aligned-spin binaries, q~U[1,10], chi~U[-0.9,0.9], log10 M~U[1,2]; start at f = 20 Hz: "
+                                  "root_find_all (r, pphi) then root_find_scalar_all (pr)", "global_batch": n},
+           "stages_ms": {"root_find_all": float(np.mean(t_rf)), "root_find_scalar_all": float(np.mean(t_sc))},
+           "batched_circular_start_ms": float(np.mean(t_circ)),
+           "finite": bool(np.isfinite(x).all() and np.isfinite(prv).all()),
+           "pr_range": [float(prv.min()), float(prv.max())]}
+    print(json.dumps(out))
+
+
 def capacity_served(p, args, budget_gb=6.0, N=1 << 12, dt=1.0 / 2048):
     """The same 16384 binaries as 64 requests of 256 arriving at once, served
     under one memory budget in two ways: admission by projected footprint (size
@@ -728,6 +793,8 @@ def main():
         return workload_imr(args)
     if args.workload == "detector":
         return workload_detector(args)
+    if args.workload == "ic":
+        return workload_ic(args)
     pipe = Pipeline(args.n, args.log2N, args.max_step, args.rtol)
     pipe.make_data()
     print(f"[bench] data ready, n={args.n} N=2^{args.log2N}", file=sys.stderr, flush=True)
