@@ -76,12 +76,13 @@ def parse():
                     help="skip timing the pass captured as one HIP graph (pipeline.GraphPass, full-length likelihood)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device to device (a kernel of known traffic, for scaling memory counters)")
-    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr", "detector", "ic"), default="uneven",
+    ap.add_argument("--workload", choices=("uneven", "spline", "capacity", "imr", "detector", "ic", "modes"), default="uneven",
                     help="uneven: the headline 1024-binary pipeline; spline: 512 synthetic (2,2) modes with 5k-60k knots "
                          "through splines + likelihood; capacity: 16384 binaries with r0 in [30, 60] M through the integrator; "
                          "imr: the headline binaries' IMR templates (five modes, NQC and ringdown) at 16384 Hz; detector: the "
                          "headline binaries' detector-frame signals in three interferometers; ic: the reference's "
-                         "initial-condition solvers as drop-ins on the headline binaries")
+                         "initial-condition solvers as drop-ins on the headline binaries; modes: the headline binaries' "
+                         "five modes through the mode-sum path and the TDInterp_wrap2 drop-in")
     return ap.parse_args()
 
 
@@ -727,6 +728,81 @@ def workload_ic(args):
     print(json.dumps(out))
 
 
+def workload_modes(args):
+    """``--workload modes``: the five modes of the headline batch through the
+    reference's mode-sum path (get_hlms and the mode splines, then the sum
+    over modes of (re + i im) e^{-i m phi} on the data's time grid):
+    k_hlm_modes_knots, the mode splines, k_resample_modes (each binary on its
+    own 2^20-sample grid at 1/4096 s) and the TDInterp_wrap2 drop-in
+    (k_td_interp2, src/interpolate.cu:441-549) on one common grid of 2^20
+    samples in seconds with its interval indices from torch.searchsorted, as
+    the reference's caller computes them.  The trajectory is integrated once
+    before the timed steps."""
+    import torch
+    from eobgpu_amd import configs, pytdinterp, pipeline as P
+    n, N, dt = args.n, 1 << args.log2N, 1.0 / 4096
+    p = configs.uneven_batch(n=n, seed=2)
+    M = p["m1"] + p["m2"]
+    b = P.EOBBatch(P.make_additional_args(p["m1"], p["m2"], p["chi1"], p["chi2"]), max_step=args.max_step)
+    b.initial_conditions(p["r0"])
+    b.integrate(rtol=args.rtol, atol=args.rtol, two_pass=True)
+    modes = P.REFERENCE_MODES
+    nm = len(modes)
+    ls = torch.tensor([l for l, _ in modes], dtype=torch.int32, device=b.device)
+    ms = torch.tensor([m for _, m in modes], dtype=torch.int32, device=b.device)
+    names = ["knots", "splines", "resample_modes", "td2_layout", "td2"]
+    times, stages = [], []
+    out = torch.empty((n, N), dtype=torch.complex128, device=b.device)
+    dataTime = torch.arange(N, dtype=torch.float64, device=b.device) * dt
+    Mt = torch.tensor(M, dtype=torch.float64, device=b.device)
+    for it in range(args.warmup + args.steps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(len(names) + 1)]
+        torch.cuda.synchronize()
+        ev[0].record()
+        b.hlm_modes_knots(modes)
+        ev[1].record()
+        coef = b.build_mode_splines()
+        ev[2].record()
+        hp, hc = b.resample_modes(M, dt, N)
+        ev[3].record()
+        # the reference's layout [bin][2M + 1][old_length] and its caller's inputs:
+        # knot times in seconds, lengths and interval indices on the common grid
+        ns = b.num_steps.to(torch.int64)
+        L = b.max_step
+        t = b.dense[:, :, 0] * (Mt * P.MTSUN_SI)[:, None]
+        valid = torch.arange(L, device=b.device)[None, :] < ns[:, None]
+        ts = torch.where(valid, t, torch.full_like(t, float("inf"))).contiguous()
+        c = coef.view(n, L, 2 * nm + 1, 4).permute(3, 0, 2, 1).contiguous()   # [4][n][2M + 1][L]
+        sc = (1.0 / (Mt * P.MTSUN_SI))[:, None, None]                            # the splines' x in seconds
+        c[1] *= sc
+        c[2] *= sc * sc
+        c[3] *= sc * sc * sc
+        tlast = ts.gather(1, (ns - 1).clamp(min=0)[:, None])[:, 0]
+        lengths = torch.clamp(torch.searchsorted(dataTime, tlast, right=True), max=N).to(torch.int32)
+        inds = (torch.searchsorted(ts, dataTime.expand(n, N).contiguous(), right=True) - 1)
+        inds = torch.minimum(inds.clamp(min=0), (ns - 2).clamp(min=0)[:, None]).to(torch.int32).contiguous()
+        ev[4].record()
+        pytdinterp.TDInterp_wrap2(out, dataTime, ts, c[0], c[1], c[2], c[3], L, ns.to(torch.int32), N, n, nm, ls, ms, inds, lengths, N,
+                                  legacy_stride=False)
+        ev[5].record()
+        torch.cuda.synchronize()
+        if it >= args.warmup:
+            times.append(ev[0].elapsed_time(ev[3]))
+            stages.append([ev[k].elapsed_time(ev[k + 1]) for k in range(len(names))])
+        del hp, hc, c, inds, ts, t
+    ms_ = float(np.mean(times))
+    st = dict(zip(names, [float(x) for x in np.mean(np.array(stages), axis=0)]))
+    res = {"metric": "five-mode templates/s (1024 uneven binaries, the reference's mode-sum path)", "value": n / (ms_ * 1e-3),
+           "unit": "templates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "1024 aligned-spin binaries (trajectory integrated once), modes (2,2),(2,1),(3,3),(4,4),(5,5) on "
+                                  "the knots, their splines and the mode sum on 2^%d samples at 1/4096 s (value: knots, splines, "
+                                  "k_resample_modes); TDInterp_wrap2 on the same grid in seconds timed beside" % args.log2N,
+                      "global_batch": n, "seq_len": N},
+           "stages_ms": st, "td2_finite": bool(torch.isfinite(torch.view_as_real(out)).all().item())}
+    print(json.dumps(res))
+
+
 def capacity_served(p, args, budget_gb=6.0, N=1 << 12, dt=1.0 / 2048):
     """The same 16384 binaries as 64 requests of 256 arriving at once, served
     under one memory budget in two ways: admission by projected footprint (size
@@ -795,6 +871,8 @@ def main():
         return workload_detector(args)
     if args.workload == "ic":
         return workload_ic(args)
+    if args.workload == "modes":
+        return workload_modes(args)
     pipe = Pipeline(args.n, args.log2N, args.max_step, args.rtol)
     pipe.make_data()
     print(f"[bench] data ready, n={args.n} N=2^{args.log2N}", file=sys.stderr, flush=True)
