@@ -364,14 +364,48 @@ struct ModeMs {
     int nm;
     int m[15];
 };
+// One sample's mode sum: row holds the interval's coefficients (Re of each
+// mode, Im of each mode, phi; 4 per series).  e^{-i m phi} is m - 1 products
+// with e^{-i phi} (the products a table of the powers would hold, without a
+// per-mode select from the table).
+template <int NMT, class Ptr>
+__device__ __forceinline__ void modes_sample(Ptr row, double x, double x2, double x3, int nm, const ModeMs& mm, double& hr,
+                                             double& hi) {
+    Ptr cp = row + 4 * (2 * nm);
+    const double ph = cp[0] + cp[1] * x + cp[2] * x2 + cp[3] * x3;
+    double sp, cph;
+    sincos(ph, &sp, &cph);
+#pragma unroll
+    for (int q = 0; q < (NMT > 0 ? NMT : nm); ++q) {
+        Ptr cr = row + 4 * q;
+        Ptr ci = row + 4 * (nm + q);
+        const double sr = cr[0] + cr[1] * x + cr[2] * x2 + cr[3] * x3;
+        const double si = ci[0] + ci[1] * x + ci[2] * x2 + ci[3] * x3;
+        double pr_ = cph, pi_ = -sp;   // e^{-i phi}
+        for (int k = 1; k < mm.m[q]; ++k) {
+            const double t = pr_ * cph + pi_ * sp;
+            pi_ = pi_ * cph - pr_ * sp;
+            pr_ = t;
+        }
+        hr += sr * pr_ - si * pi_;
+        hi += sr * pi_ + si * pr_;
+    }
+}
+
+// NMT > 0: built for that many modes (the loop over modes unrolled); 0: any.
+// When all lanes of a wavefront sample one knot interval, its coefficient row
+// is read once through the scalar cache (constant address space: the table
+// is read-only while the kernel runs).
+template <int NMT>
 __global__ __launch_bounds__(kBlock) void k_resample_modes(const double* __restrict__ dense, const double* __restrict__ coef,
                                                            const int* __restrict__ num_steps, const double* __restrict__ amp,
                                                            const double* __restrict__ dT_M, int max_step, int N, ModeMs mm,
                                                            double* __restrict__ hplus, double* __restrict__ hcross,
                                                            int* __restrict__ nsamp) {
+    typedef const __attribute__((address_space(4))) double* ConstPtr;
     const int b = blockIdx.y;
     const int len = num_steps[b];
-    const int nm = mm.nm, S = 2 * nm + 1, cs = 4 * S;
+    const int nm = NMT > 0 ? NMT : mm.nm, S = 2 * nm + 1, cs = 4 * S;
     double* out = hplus + (size_t)b * N;
     double* outc = hcross + (size_t)b * N;
     if (len < 2) {
@@ -405,33 +439,13 @@ __global__ __launch_bounds__(kBlock) void k_resample_modes(const double* __restr
             if (j < nb) {
                 const double tj = t0 + (double)j * dT;
                 const int ind = knot_interval<kBlock>(win, T, 5, len, tj);
-                const double x = tj - T[5 * ind];
-                const double x2 = x * x, x3 = x2 * x;
-                const double* c = C + (size_t)cs * ind;
-                const double* cp = c + 4 * (2 * nm);
-                const double ph = cp[0] + cp[1] * x + cp[2] * x2 + cp[3] * x3;
-                double sp, cph;
-                sincos(ph, &sp, &cph);
-                // e^{-i k phi}, k = 1 .. 8
-                double er[kMaxModeM + 1], ei[kMaxModeM + 1];
-                er[0] = 1.0; ei[0] = 0.0;
-#pragma unroll
-                for (int k = 1; k <= kMaxModeM; ++k) {
-                    er[k] = er[k - 1] * cph + ei[k - 1] * sp;
-                    ei[k] = ei[k - 1] * cph - er[k - 1] * sp;
-                }
-                for (int q = 0; q < nm; ++q) {
-                    const double* cr = c + 4 * q;
-                    const double* ci = c + 4 * (nm + q);
-                    const double sr = cr[0] + cr[1] * x + cr[2] * x2 + cr[3] * x3;
-                    const double si = ci[0] + ci[1] * x + ci[2] * x2 + ci[3] * x3;
-                    const int k = mm.m[q];
-                    double pr_ = 1.0, pi_ = 0.0;
-#pragma unroll
-                    for (int kk = 1; kk <= kMaxModeM; ++kk)
-                        if (kk == k) { pr_ = er[kk]; pi_ = ei[kk]; }
-                    hr += sr * pr_ - si * pi_;
-                    hi += sr * pi_ + si * pr_;
+                const int i0 = __builtin_amdgcn_readfirstlane(ind);
+                if (__all(ind == i0)) {
+                    const double x = tj - ((ConstPtr)T)[5 * (size_t)i0];
+                    modes_sample<NMT>((ConstPtr)(C + (size_t)cs * i0), x, x * x, x * x * x, nm, mm, hr, hi);
+                } else {
+                    const double x = tj - T[5 * (size_t)ind];
+                    modes_sample<NMT>(C + (size_t)cs * ind, x, x * x, x * x * x, nm, mm, hr, hi);
                 }
                 hr *= a;
                 hi *= a;
@@ -718,8 +732,16 @@ int eobhip_resample_modes(const double* dense, const double* coef, const int* nu
     if (n == 0) return EOBHIP_OK;
     int bx = (int)((N + (long long)kResSpan - 1) / kResSpan);
     if (bx > 4096) bx = 4096;
-    hipLaunchKernelGGL(k_resample_modes, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M, max_step, N,
-                       mm, hplus, hcross, nsamp);
+    // built for the reference's five modes and for (2,2) alone; any other count runs the general build
+    if (nm == 5)
+        hipLaunchKernelGGL(k_resample_modes<5>, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M,
+                           max_step, N, mm, hplus, hcross, nsamp);
+    else if (nm == 1)
+        hipLaunchKernelGGL(k_resample_modes<1>, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M,
+                           max_step, N, mm, hplus, hcross, nsamp);
+    else
+        hipLaunchKernelGGL(k_resample_modes<0>, dim3(bx, n), dim3(kBlock), 0, as_stream(stream), dense, coef, num_steps, amp, dT_M,
+                           max_step, N, mm, hplus, hcross, nsamp);
     return check_launch("resample_modes");
 }
 

@@ -132,31 +132,33 @@ def test_modes_on_knots_match_oracle(oracle, dev):
 
 def test_mode_splines_and_sum_match_scipy(oracle, dev):
     """Splines of the 11 series and the mode sum on the uniform grid: scipy's
-    not-a-knot CubicSpline of each series, h = sum_j S_j e^{-i m_j phi}."""
+    not-a-knot CubicSpline of each series, h = sum_j S_j e^{-i m_j phi}; with
+    the reference's five modes (a build of the sum for five modes) and with
+    three (the build for any count)."""
     p, add, b = _batch_with_trajectories(oracle, dev, n=5, seed=8, r0=12.0)
-    modes = P.REFERENCE_MODES
-    ms = [m for _, m in modes]
-    nm = len(modes)
-    b.hlm_modes_knots(modes)
-    b.build_mode_splines()
     Msun, dt, N = 30.0, 1.0 / 4096, 1 << 15
-    hp, hc = b.resample_modes(np.full(b.n, Msun), dt, N, amp=1.5)
-    hp, hc = hp.cpu().numpy(), hc.cpu().numpy()
-    kn = b.mode_knots.cpu().numpy()
-    dense = b.dense.cpu().numpy()
-    ns = b.num_steps.cpu().numpy()
     dT = dt / (Msun * P.MTSUN_SI)
-    for i in range(b.n):
-        k = ns[i]
-        t = dense[i, :k, 0]
-        nsamp = min(N, int((t[-1] - t[0]) / dT) + 1)
-        tj = t[0] + np.arange(nsamp) * dT
-        S = [CubicSpline(t, kn[i, :k, q], bc_type="not-a-knot")(tj) for q in range(2 * nm + 1)]
-        h = sum((S[j] + 1j * S[nm + j]) * np.exp(-1j * ms[j] * S[2 * nm]) for j in range(nm)) * 1.5
-        scale = np.abs(h).max()
-        assert np.max(np.abs(hp[i, :nsamp] - h.real)) < 1e-9 * scale, i
-        assert np.max(np.abs(hc[i, :nsamp] - h.imag)) < 1e-9 * scale, i
-        assert not hp[i, nsamp:].any() and not hc[i, nsamp:].any()
+    for modes in (P.REFERENCE_MODES, ((2, 2), (3, 3), (2, 1))):
+        ms = [m for _, m in modes]
+        nm = len(modes)
+        b.hlm_modes_knots(modes)
+        b.build_mode_splines()
+        hp, hc = b.resample_modes(np.full(b.n, Msun), dt, N, amp=1.5)
+        hp, hc = hp.cpu().numpy(), hc.cpu().numpy()
+        kn = b.mode_knots.cpu().numpy()
+        dense = b.dense.cpu().numpy()
+        ns = b.num_steps.cpu().numpy()
+        for i in range(b.n):
+            k = ns[i]
+            t = dense[i, :k, 0]
+            nsamp = min(N, int((t[-1] - t[0]) / dT) + 1)
+            tj = t[0] + np.arange(nsamp) * dT
+            S = [CubicSpline(t, kn[i, :k, q], bc_type="not-a-knot")(tj) for q in range(2 * nm + 1)]
+            h = sum((S[j] + 1j * S[nm + j]) * np.exp(-1j * ms[j] * S[2 * nm]) for j in range(nm)) * 1.5
+            scale = np.abs(h).max()
+            assert np.max(np.abs(hp[i, :nsamp] - h.real)) < 1e-9 * scale, (nm, i)
+            assert np.max(np.abs(hc[i, :nsamp] - h.imag)) < 1e-9 * scale, (nm, i)
+            assert not hp[i, nsamp:].any() and not hc[i, nsamp:].any()
     # the (2,2)-only sum is the (2,2) pipeline's h+ and hx
     b.hlm_modes_knots(((2, 2),))
     b.build_mode_splines()
