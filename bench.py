@@ -849,7 +849,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched path shards by binary and "
-              "this build measures the one-GPU share.", file=sys.stderr)
+              "this benchmark measures one GPU.", file=sys.stderr)
         sys.exit(2)
     import torch
     torch.cuda.init()
@@ -955,7 +955,7 @@ def main():
     plain = None
     if not args.no_plain:
         # the reference's batching model: one launch per attempted step, one lane per
-        # binary; and the same per-step launches with this build's 64-lane groups
+        # binary; and the same per-step launches with this library's 64-lane groups
         plain = {}
         for name, grp in (("lane_per_binary", args.plain_group), ("group64", 64)):
             wall_p, stages_p = time_pipeline(pipe, "plain", grp, args.plain_steps, 0)

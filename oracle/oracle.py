@@ -5,7 +5,7 @@ numpy/ctypes front end for the CPU restatement in ``oracle_eob.cpp`` (built into
 ``src/DOPR853.cu`` compiled unmodified into ``oracle/_ref/libdopr853_ref.so``.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg
-import this module, and only as the checker / CPU baseline.  The shipped
+import this module, and only as the reference results they compare against / the CPU baseline.  The shipped
 ``eobgpu_amd`` package never imports it.
 """
 from __future__ import annotations

@@ -1,6 +1,6 @@
 // Oracle test infrastructure (CPU only). Not part of the shipped library:
 // only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
-// and only as the checker.
+// and only as the results to compare against.
 //
 // Forward-mode automatic differentiation with a fixed number of tangents.
 // Nesting Dual<Dual<double,N>,N> gives exact second derivatives (Hessian).

@@ -1,6 +1,6 @@
 // ORACLE — test infrastructure only.  A plain CPU restatement of the
 // reference's batched SEOBNR hot path, used by tests/ (and by bench.py's
-// cpu_baseline leg and __graft_entry__.smoke()) as the checker.  Nothing in the
+// cpu_baseline leg and __graft_entry__.smoke()) as the results to compare against.  Nothing in the
 // shipped eobgpu_amd library links, loads or calls this file.
 //
 // Each function cites the reference file:line it restates.  Parity pins (what

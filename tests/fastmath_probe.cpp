@@ -1,8 +1,8 @@
 // Host build of the right-hand side's arithmetic helpers
 // (eobgpu_amd/csrc/eob_fastmath.hpp).  log_pos writes every multiply-add as
-// fma() with contraction off, so this build rounds as the device build does;
+// fma() with contraction off, so this host build rounds as the device build does;
 // the division and square-root helpers start here from single-precision
-// estimates (coarser than the hardware's), so this build tests their refinement.
+// estimates (coarser than the hardware's), so this host build tests their refinement.
 #include "../eobgpu_amd/csrc/eob_fastmath.hpp"
 
 extern "C" void probe_log_pos(const double* x, double* y, int n) {
